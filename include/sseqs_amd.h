@@ -1,0 +1,203 @@
+/*
+ * sseqs_amd.h -- C ABI of the MI355X (gfx950) MSA-search library
+ * (sseqs_amd/libsseqs_amd.so).
+ *
+ * Conventions shared by every entry point:
+ *   - All pointers are device pointers unless the parameter name starts with
+ *     "h_".  Nothing is synchronised: work is enqueued on `stream`.
+ *   - The return value is 0 on success.  A negative value is an argument
+ *     error detected on the host (nothing was launched); a positive value is
+ *     a hipError_t from a launch.  sseqs_last_error() describes the last
+ *     failure on the calling thread.  The reference skipped unsupported
+ *     sizes silently (sseqs/sw.cu:357-369, sseqs/filter.cu:320); these
+ *     entry points report them instead.
+ *   - fp16 buffers are IEEE binary16 bit patterns (uint16_t).
+ *   - Sizes are int64_t where the reference used int, so one call can cover
+ *     an HBM-resident database of more than 2^31 symbols.
+ *
+ * Entry points that replace a reference launcher keep its name and argument
+ * order, adding the stream where the reference had none; each cites the
+ * launcher it replaces.  The sseqs_* entry points have no single reference
+ * counterpart: they are the database-resident forms of the same operators.
+ */
+#ifndef SSEQS_AMD_H
+#define SSEQS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Message for the last non-zero return on this thread ("" if none). */
+const char* sseqs_last_error(void);
+/* Library ABI version (bumped when a signature changes). */
+int sseqs_abi_version(void);
+
+/* ---------------------------------------------------------------- xbit -- */
+
+/* Unpack n_syms 5-bit symbols starting at byte `lo` of d_in into ASCII
+ * (symbol + 64) at d_out[0..n_syms).
+ * Replaces decompress_launcher (sseqs/xbit.cu:65-71), which ran on the
+ * legacy default stream and wrote whole 8-symbol groups past n_syms. */
+int decompress_launcher(const uint8_t* d_in, uint8_t* d_out, int64_t n_syms, int64_t lo,
+                        hipStream_t stream);
+
+/* Pack n_syms ASCII symbols into (5*n_syms+7)/8 bytes, MSB first.
+ * Replaces compress_launcher (sseqs/xbit.cu:73-78). */
+int compress_launcher(const uint8_t* d_in, uint8_t* d_out, int64_t n_syms, hipStream_t stream);
+
+/* ------------------------------------------------------ prefilter (conv) -- */
+
+/* Low-rank FFT-correlation prefilter score of B segments of `flat` (ASCII),
+ * segment b = flat[offsets[b] .. offsets[b]+lengths[b]).  corr[b] (fp16) is
+ * overwritten with max over even m of the N-point circular correlation
+ * 0.5 * sum_r sum_t A_r[qrev[t]] * B_r[code(seg[(m-t) mod N])]
+ * with A = weight_LUTS_A (k rows of vocab_size floats, query side) and
+ * B = W_db (database side, read as a flat array: code 21 reads the next
+ * row's column 0, so W_db must hold k*vocab_size+1 floats).
+ * Replaces launch_warp_corr (sseqs/filter.cu:161-174, kernels
+ * sseqs/filter.cu:92-159 and sseqs/fft.cu:17-415).  out_stats and
+ * query_rfft are accepted for signature compatibility and not used.
+ * N may be any power of two from 16 to 2048 (the reference handled 64..1024
+ * and left scores at 0 for 2048). */
+int launch_warp_corr(uint16_t* corr, const uint8_t* query_reversed_indices,
+                     const float* weight_LUTS_A, const float* W_db, const uint8_t* flat,
+                     const int32_t* offsets, const int32_t* lengths, float* out_stats, int B,
+                     int vocab_size, int k, int query_len, int maxlen, int N_full,
+                     hipStream_t stream, uint16_t* query_rfft);
+
+/* ---------------------------------------------------------- Smith-Waterman -- */
+
+/* Linear-gap local alignment score of M targets against the query
+ * (codes 0..19, length 1..1024).  Target g is
+ * ascii[P[good_idx[g]-1]+1 .. P[good_idx[g]]) with P = `lengths` (the
+ * reference's name for the delimiter-position array); P[-1] is taken as -1.
+ * out[g] = -1 for an empty query or target, like the reference.
+ * Scores follow the reference's 32-stripe rule exactly (no horizontal move
+ * into a column j with j % ceil(Lq/32) == 0).  gap_val must be <= 0.
+ * Replaces launch_sw_cuda_blosum62 (sseqs/sw.cu:336-370). */
+int launch_sw_cuda_blosum62(const uint8_t* query_seq_indices_ptr, int query_length,
+                            const int32_t* good_idx, const uint8_t* ascii, const int32_t* lengths,
+                            int num_sequences_in_db, int32_t* output_scores_ptr, int gap_val,
+                            hipStream_t stream);
+
+/* Affine-gap (open gap_open, extend gap_extend, both >= 0) counterpart with
+ * the same target selection and stripe rule.
+ * Replaces launch_sw_cuda_affine (sseqs/sw.cu:372-425). */
+int launch_sw_cuda_affine(const uint8_t* query_seq_indices_ptr, int query_length,
+                          const int32_t* good_idx, const uint8_t* ascii, const int32_t* lengths,
+                          int num_sequences_in_db, int32_t* output_scores_ptr, int gap_open,
+                          int gap_extend, hipStream_t stream);
+
+/* ------------------------------------------------ HBM-resident database -- */
+
+/* Index of a packed database holding n_syms symbols: every maximal run of
+ * non-'@' symbols is one sequence.  Writes up to `cap` entries of
+ * seq_off (symbol offset of the first residue) and seq_len, and the number
+ * of sequences found to *d_nseq (device int64).  Empty runs (consecutive
+ * '@') are skipped, matching the `lengths != 0` filter of sseqs/msa.py:253.
+ * ws must hold sseqs_db_index_ws_bytes(n_syms) bytes.
+ * Database-resident form of the delimiter scan of sseqs/msa.py:227-267. */
+size_t sseqs_db_index_ws_bytes(int64_t n_syms);
+int sseqs_db_index(const uint8_t* packed, int64_t n_syms, int64_t* seq_off, int32_t* seq_len,
+                   int64_t cap, int64_t* d_nseq, void* ws, size_t ws_bytes, hipStream_t stream);
+
+/* launch_warp_corr's score for sequences s in [s_begin, s_end) of the
+ * resident database (symbols unpacked on the fly from `packed`), all with
+ * correlation size N, written as fp32 to scores[s - s_begin].  qrev (codes
+ * 0..20), lutA and lutB (k rows of vocab_size floats, flat indexing as in
+ * launch_warp_corr) are device arrays.  Sequences longer than
+ * min(maxlen, N) are truncated to it, as in sseqs/fft.cu:124-126.
+ * N: power of two, 16..2048; k: 1..20.  Fused unpack + prefilter of the
+ * resident database. */
+int sseqs_db_prefilter(const uint8_t* packed, const int64_t* seq_off, const int32_t* seq_len,
+                       int64_t s_begin, int64_t s_end, const uint8_t* qrev, int query_len,
+                       const float* lutA, const float* lutB, int vocab_size, int k, int N,
+                       int maxlen, float* scores, hipStream_t stream);
+
+/* Hit gather: row i of out (n rows of `width` bytes) receives the first
+ * min(seq_len[ids[i]], width) residues of database sequence ids[i] as
+ * ASCII, and out_len[i] that count.  Database-resident form of the hit
+ * gather of sseqs/msa.py:349-370 (width 500 there, sseqs/msa.py:183). */
+int sseqs_db_gather(const uint8_t* packed, const int64_t* seq_off, const int32_t* seq_len,
+                    const int64_t* ids, int64_t n, int width, uint8_t* out, int32_t* out_len,
+                    hipStream_t stream);
+
+/* Affine score (reference stripe rule) of database sequences cand[0..n) of
+ * the resident database against the query (codes 0..19, length 1..1024).
+ * out[i] = -1 for an empty sequence. */
+int sseqs_db_sw_affine(const uint8_t* packed, const int64_t* seq_off, const int32_t* seq_len,
+                       const int64_t* cand, int64_t n, const uint8_t* query, int query_len,
+                       int gap_open, int gap_extend, int32_t* out, hipStream_t stream);
+
+/* ---------------------------------------------------------------- top-k -- */
+
+/* dtype codes for sseqs_topk */
+#define SSEQS_F32 0
+#define SSEQS_F16 1
+#define SSEQS_I32 2
+
+/* Indices of the k largest of n scores, ties broken by the smaller index
+ * (torch.topk leaves tie order unspecified; this order is the defined one).
+ * sorted != 0: out_idx[0..k) in (score desc, index asc) order, k <= 16384;
+ * sorted == 0: out_idx[0..k) in ascending index order, any k <= n.
+ * NaN ranks below every number.  ws: sseqs_topk_ws_bytes(n) bytes. */
+size_t sseqs_topk_ws_bytes(int64_t n);
+int sseqs_topk(const void* scores, int dtype, int64_t n, int64_t k, int sorted, int64_t* out_idx,
+               void* ws, size_t ws_bytes, hipStream_t stream);
+
+/* ------------------------------------------------------------ backtrack -- */
+
+/* Full affine DP (gaps open from H, -5 for non-residues) and stateless
+ * traceback of B targets (ASCII, target b = targets[t_off[b] .. +t_len[b]))
+ * against the ASCII query, with the semantics of the Triton kernels
+ * sseqs/sw.py:408-768 (driven by sw_affine_backtrack, sseqs/sw.py:27-219).
+ * Per target b, with A = max_align:
+ *   out_q[b*A + A-len .. b*A + A) / out_t[...]: aligned query / target
+ *   (ASCII, '-' for gaps), right-aligned like the reference;
+ *   info[b*6 + 0..5] = {len, q_start, q_end, t_start, t_end, score}
+ *   (starts/ends -1 and len 0 when nothing aligns).
+ * max_align must be >= query_len + max(t_len) + 1.
+ * cell_off[b] = query_len * sum_{b' < b} t_len[b'] and total_cells is the
+ * full sum (host value); ws: sseqs_backtrack_ws_bytes(query_len,
+ * total_cells, B) bytes. */
+size_t sseqs_backtrack_ws_bytes(int query_len, int64_t total_cells, int B);
+int sseqs_backtrack(const uint8_t* query, int query_len, const uint8_t* targets,
+                    const int64_t* t_off, const int32_t* t_len, const int64_t* cell_off,
+                    int64_t total_cells, int B, int gap_open, int gap_extend, int max_align,
+                    uint8_t* out_q, uint8_t* out_t, int32_t* info, void* ws, size_t ws_bytes,
+                    hipStream_t stream);
+
+/* --------------------------------------------------- synthetic database -- */
+
+/* Parameters of the synthetic generator; layout shared with
+ * sseqs_amd/synth.py (1712 bytes). */
+typedef struct {
+    uint64_t seed;
+    int32_t bg_cdf[20];
+    int32_t sub_cdf[400];
+    int32_t p_indel, p_ins, p_ext, id_lo, id_span;
+    int32_t hom_period;
+} sseqs_gen_params;
+
+/* Length-sorted synthetic database of n sequences.  Length classes:
+ * cls_start[K+1] (cumulative sequence counts), cls_len[K], cls_sym[K]
+ * (symbol offset of each class).  Queries (codes 0..19) concatenated with
+ * q_off[nq+1] are planted as mutated homologs, one per block of
+ * hom_period sequences.  Writes packed (ceil(5*n_syms/8) bytes), seq_off,
+ * seq_len and hom_query (query id of a homolog, -1 otherwise).
+ * ws: sseqs_gen_ws_bytes(n, hom_period) bytes. */
+size_t sseqs_gen_ws_bytes(int64_t n, int hom_period);
+int sseqs_gen_db(const sseqs_gen_params* h_params, const uint8_t* qcodes, const int64_t* q_off,
+                 int nq, const int64_t* cls_start, const int32_t* cls_len, const int64_t* cls_sym,
+                 int K, int64_t n, int64_t n_syms, uint8_t* packed, int64_t* seq_off,
+                 int32_t* seq_len, int32_t* hom_query, void* ws, size_t ws_bytes,
+                 hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SSEQS_AMD_H */

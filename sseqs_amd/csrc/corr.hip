@@ -1,0 +1,240 @@
+// Low-rank correlation prefilter on MFMA.
+//
+// The reference (sseqs/filter.cu:92-159, sseqs/fft.cu:17-415) computes, per
+// database segment d and rank r < k, an N-point real FFT of
+//   b_r[t] = LUT_B[r][code(d_t)]   (t < min(len, maxlen, N), else 0)
+// multiplies it by the FFT of the reversed, LUT_A-weighted query
+//   a_r[t] = LUT_A[r][qrev[t]]     (t < min(Lq, N), else 0)
+// inverts, sums the ranks and keeps max over even m of (1/N) * (H-point
+// unnormalised inverse), i.e.
+//   score = max_{m even} 0.5 * sum_r sum_p a_r[(m-p) mod N] * b_r[p].
+//
+// Here that sum is evaluated directly as a matrix product: for a tile of 32
+// segments (MFMA columns) and 32 even shifts m = 2i (MFMA rows),
+//   C[i][s] = sum_{r,p} T_r[i][p] * b_r^s[p],   T_r[i][p] = 0.5 a_r[(2i-p) mod N]
+// with K = (r, p) in steps of 16 on v_mfma_f32_32x32x16_f16.  T is Toeplitz,
+// so one LDS row per rank, qtab[r][u] = 0.5 a_r[(-u) mod N] (extended by 8
+// entries to avoid wrap handling), yields every A fragment with one 16-byte
+// read.  B fragments are table lookups of the staged residue codes.  Inputs
+// are fp16, accumulation fp32, so there is no fp16 overflow (the reference's
+// all-fp16 evaluation saturates to inf on strong hits).
+#include "common.h"
+
+namespace sseqs {
+
+typedef _Float16 half8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int CORR_PAD = 39;   // code index whose table value is 0
+constexpr int CORR_NB = 40;    // table entries per rank
+constexpr int CORR_MT = 8;     // m-tiles (32 shifts each) accumulated per pass
+
+// Segments of an ASCII batch: offsets/lengths as in sseqs/msa.py:266-287.
+struct CorrAscii {
+    const uint8_t* flat;
+    const int32_t* off;
+    const int32_t* len;
+    __device__ int64_t offset(int64_t b) const { return off[b]; }
+    __device__ int length(int64_t b) const { return len[b]; }
+    // table index: prefilter code 0..21 (sseqs/fft.cu:5-15)
+    __device__ uint32_t code(int64_t o, int p) const { return (uint32_t)corr_code(flat[o + p]); }
+};
+
+// Sequences of the HBM-resident packed database.
+struct CorrPacked {
+    const uint8_t* packed;
+    const int64_t* off;
+    const int32_t* len;
+    __device__ int64_t offset(int64_t b) const { return off[b]; }
+    __device__ int length(int64_t b) const { return len[b]; }
+    // table index: the 5-bit symbol itself (the table maps it through the code)
+    __device__ uint32_t code(int64_t o, int p) const { return packed_sym(packed, o + p) - 64u; }
+};
+
+template <class OutT>
+__device__ inline void store_score(OutT* out, int64_t i, float v);
+template <>
+__device__ inline void store_score<float>(float* out, int64_t i, float v) { out[i] = v; }
+template <>
+__device__ inline void store_score<_Float16>(_Float16* out, int64_t i, float v) {
+    out[i] = (_Float16)v;
+}
+
+// One wave (64 lanes) per block; the block walks tiles of 32 segments.
+// Dynamic LDS: qtab k*(N+8) fp16 | btab k*40 fp16 | codes 32*(KPmax+8) bytes.
+template <class Src, class OutT, bool PACKED_CODES>
+__global__ __launch_bounds__(64) void k_corr(Src src, int64_t s_begin, int64_t nseq,
+                                             const uint8_t* __restrict__ qrev, int Lq,
+                                             const float* __restrict__ lutA,
+                                             const float* __restrict__ lutB, int V, int k, int N,
+                                             int maxlen, int KPmax, OutT* __restrict__ out) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    _Float16* qtab = reinterpret_cast<_Float16*>(smem);
+    const int QS = N + 8;
+    _Float16* btab = qtab + k * QS;
+    uint8_t* codes = reinterpret_cast<uint8_t*>(btab + k * CORR_NB);
+    const int KPS = KPmax + 8;  // row stride of the code tile (bank spread)
+
+    const int lane = threadIdx.x;
+    const int Lqe = min(Lq, N);
+    // query table: qtab[r][u] = 0.5 * A[r][qrev[(-u) mod N]]
+    for (int idx = lane; idx < k * QS; idx += 64) {
+        int r = idx / QS, u = idx - r * QS;
+        int uu = u & (N - 1);
+        int t = (N - uu) & (N - 1);
+        float v = 0.f;
+        if (t < Lqe) v = 0.5f * lutA[r * V + (int)qrev[t]];
+        qtab[idx] = (_Float16)v;
+    }
+    // database table, flat indexing as the reference (code 21 -> next row col 0)
+    for (int idx = lane; idx < k * CORR_NB; idx += 64) {
+        int r = idx / CORR_NB, c = idx - r * CORR_NB;
+        float v = 0.f;
+        if (PACKED_CODES) {
+            if (c < 32) v = lutB[r * V + corr_code((uint32_t)c + 64u)];
+        } else {
+            if (c < 22) v = lutB[r * V + c];
+        }
+        btab[idx] = (_Float16)v;
+    }
+
+    const int sl = lane & 31, h = lane >> 5;
+    const int NMT = N >= 64 ? N / 64 : 1;
+    for (int64_t tile = blockIdx.x; tile * 32 < nseq; tile += gridDim.x) {
+        const int64_t t0 = tile * 32;
+        const int64_t sid = t0 + sl;
+        const bool in = sid < nseq;
+        int Le = 0;
+        int64_t o = 0;
+        if (in) {
+            o = src.offset(s_begin + sid);
+            Le = max(0, min(min(src.length(s_begin + sid), maxlen), N));
+        }
+        int kp = Le;
+#pragma unroll
+        for (int off = 16; off > 0; off >>= 1) kp = max(kp, __shfl_xor(kp, off, 64));
+        kp = (kp + 15) & ~15;
+        __syncthreads();  // codes of the previous tile are consumed; tables are ready
+        // stage codes: lane (sl, h) fills half of row sl, 4 codes per LDS store
+        {
+            const int hw = kp >> 1;
+            const int p0 = h * hw;
+            uint32_t* row = reinterpret_cast<uint32_t*>(codes + sl * KPS + p0);
+            for (int p = 0; p < hw; p += 4) {
+                uint32_t w = 0;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    int pp = p0 + p + b;
+                    uint32_t c = pp < Le ? src.code(o, pp) : (uint32_t)CORR_PAD;
+                    w |= c << (8 * b);
+                }
+                row[p >> 2] = w;
+            }
+        }
+        __syncthreads();
+
+        float best = -INFINITY;
+        for (int mp = 0; mp < NMT; mp += CORR_MT) {
+            f32x16 acc[CORR_MT];
+#pragma unroll
+            for (int t = 0; t < CORR_MT; ++t)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) acc[t][e] = 0.f;
+            for (int r = 0; r < k; ++r) {
+                const _Float16* qr = qtab + r * QS;
+                const _Float16* br = btab + r * CORR_NB;
+                for (int p0 = 0; p0 < kp; p0 += 16) {
+                    const int pb = p0 + 8 * h;
+                    uint64_t cw = *reinterpret_cast<const uint64_t*>(codes + sl * KPS + pb);
+                    half8 bfrag;
+#pragma unroll
+                    for (int x = 0; x < 8; ++x) bfrag[x] = br[(cw >> (8 * x)) & 0xff];
+#pragma unroll
+                    for (int t = 0; t < CORR_MT; ++t) {
+                        if (mp + t < NMT) {
+                            const int i = (mp + t) * 32 + sl;
+                            const int u0 = (pb - 2 * i) & (N - 1);
+                            const uint32_t* q32 = reinterpret_cast<const uint32_t*>(qr + u0);
+                            union {
+                                uint32_t w[4];
+                                half8 v;
+                            } a;
+                            a.w[0] = q32[0];
+                            a.w[1] = q32[1];
+                            a.w[2] = q32[2];
+                            a.w[3] = q32[3];
+                            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a.v, bfrag, acc[t], 0,
+                                                                            0, 0);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < CORR_MT; ++t) {
+                if (mp + t < NMT) {
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) best = fmaxf(best, acc[t][e]);
+                }
+            }
+        }
+        // lanes sl and sl+32 hold the two row halves of column sl
+        best = fmaxf(best, __shfl_xor(best, 32, 64));
+        if (h == 0 && in) store_score<OutT>(out, sid, best);
+    }
+}
+
+template <class Src, class OutT, bool PACKED_CODES>
+static int launch_corr(const Src& src, int64_t s_begin, int64_t nseq, const uint8_t* qrev, int Lq,
+                       const float* lutA, const float* lutB, int V, int k, int N, int maxlen,
+                       OutT* out, hipStream_t stream) {
+    if (nseq <= 0) return 0;
+    if (N < 16 || N > 2048 || (N & (N - 1)))
+        return fail(-1, "prefilter: N must be a power of two in 16..2048");
+    if (k < 1 || k > 20) return fail(-1, "prefilter: k must be 1..20");
+    if (V < 1) return fail(-1, "prefilter: vocab_size must be >= 1");
+    int mlen = maxlen < 0 ? 0 : maxlen;
+    int KPmax = (min(mlen, N) + 15) & ~15;
+    size_t lds = (size_t)k * (N + 8) * 2 + (size_t)k * CORR_NB * 2 + (size_t)32 * (KPmax + 8);
+    lds = (lds + 15) & ~(size_t)15;
+    if (lds > 160 * 1024) return fail(-1, "prefilter: tables exceed LDS (reduce k or N)");
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_corr<Src, OutT, PACKED_CODES>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return fail(-3, "prefilter: cannot enable large LDS");
+    int64_t tiles = ceil_div(nseq, 32);
+    unsigned grid = (unsigned)(tiles < 8192 ? tiles : 8192);
+    hipLaunchKernelGGL((k_corr<Src, OutT, PACKED_CODES>), dim3(grid), dim3(64), lds, stream, src,
+                       s_begin, nseq, qrev, Lq, lutA, lutB, V, k, N, mlen, KPmax, out);
+    return check_launch("prefilter");
+}
+
+}  // namespace sseqs
+
+using namespace sseqs;
+
+extern "C" int launch_warp_corr(uint16_t* corr, const uint8_t* qrev, const float* lutA,
+                                const float* W_db, const uint8_t* flat, const int32_t* offsets,
+                                const int32_t* lengths, float* out_stats, int B, int vocab_size,
+                                int k, int query_len, int maxlen, int N_full, hipStream_t stream,
+                                uint16_t* query_rfft) {
+    (void)out_stats;
+    (void)query_rfft;
+    if (B < 0) return fail(-1, "warp_corr: negative segment count");
+    if (B == 0) return 0;
+    CorrAscii src{flat, offsets, lengths};
+    return launch_corr<CorrAscii, _Float16, false>(src, 0, B, qrev, query_len, lutA, W_db,
+                                                   vocab_size, k, N_full, maxlen,
+                                                   reinterpret_cast<_Float16*>(corr), stream);
+}
+
+extern "C" int sseqs_db_prefilter(const uint8_t* packed, const int64_t* seq_off,
+                                  const int32_t* seq_len, int64_t s_begin, int64_t s_end,
+                                  const uint8_t* qrev, int query_len, const float* lutA,
+                                  const float* lutB, int vocab_size, int k, int N, int maxlen,
+                                  float* scores, hipStream_t stream) {
+    if (s_end < s_begin) return fail(-1, "db_prefilter: s_end < s_begin");
+    CorrPacked src{packed, seq_off, seq_len};
+    return launch_corr<CorrPacked, float, true>(src, s_begin, s_end - s_begin, qrev, query_len,
+                                                lutA, lutB, vocab_size, k, N, maxlen, scores,
+                                                stream);
+}

@@ -1,0 +1,279 @@
+// Deterministic device top-k.
+//
+// Used wherever the reference calls torch.topk / torch.sort on the search
+// path (sseqs/msa.py:297, 307, 324, 335).  Scores become order-preserving
+// 32-bit keys; three histogram passes (11 + 11 + 10 bits, most significant
+// first) find the k-th largest key T and how many keys equal to T are taken;
+// a compaction pass then writes every index with key > T plus the first
+// ones with key == T, in ascending index order.  The optional sort orders
+// them by (key desc, index asc) with a bitonic sort in LDS.  The tie rule
+// (smaller index first) is the build's definition; torch.topk leaves it open.
+#include "common.h"
+
+namespace sseqs {
+
+constexpr int SEL_BINS = 2048;
+constexpr int SEL_TILE = 4096;      // elements per block in count/write passes
+constexpr int SEL_SORT_MAX = 16384;
+
+struct SelState {
+    uint32_t prefix, mask, k_rem, pad;
+};
+
+__device__ inline uint32_t key_f32(float x) {
+    if (x != x) return 0u;  // NaN ranks lowest
+    uint32_t b = __float_as_uint(x);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+
+__device__ inline uint32_t load_key(const void* s, int dtype, int64_t i) {
+    if (dtype == SSEQS_F32) return key_f32(static_cast<const float*>(s)[i]);
+    if (dtype == SSEQS_F16) return key_f32((float)static_cast<const _Float16*>(s)[i]);
+    return (uint32_t) static_cast<const int32_t*>(s)[i] ^ 0x80000000u;
+}
+
+__global__ void k_sel_init(uint32_t* hist, SelState* st, uint32_t k) {
+    for (int i = threadIdx.x; i < SEL_BINS; i += blockDim.x) hist[i] = 0;
+    if (threadIdx.x == 0) {
+        st->prefix = 0;
+        st->mask = 0;
+        st->k_rem = k;
+        st->pad = 0;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sel_hist(const void* s, int dtype, int64_t n,
+                                                  const SelState* st, uint32_t* hist, int shift,
+                                                  int bits) {
+    __shared__ uint32_t lh[SEL_BINS];
+    for (int i = threadIdx.x; i < SEL_BINS; i += 256) lh[i] = 0;
+    __syncthreads();
+    const uint32_t prefix = st->prefix, mask = st->mask, bm = (1u << bits) - 1u;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
+        uint32_t key = load_key(s, dtype, i);
+        if ((key & mask) == prefix) atomicAdd(&lh[(key >> shift) & bm], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < SEL_BINS; i += 256)
+        if (lh[i]) atomicAdd(&hist[i], lh[i]);
+}
+
+// One block: walk bins from the top, fix the next digit of T, clear hist.
+__global__ __launch_bounds__(1024) void k_sel_pick(SelState* st, uint32_t* hist, int shift,
+                                                   int bits) {
+    __shared__ uint32_t cnt[SEL_BINS];
+    const int nb = 1 << bits;
+    for (int i = threadIdx.x; i < SEL_BINS; i += 1024) {
+        cnt[i] = i < nb ? hist[i] : 0u;
+        hist[i] = 0u;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t need = st->k_rem, above = 0;
+        int b = nb - 1;
+        for (; b > 0; --b) {
+            if (above + cnt[b] >= need) break;
+            above += cnt[b];
+        }
+        st->prefix |= (uint32_t)b << shift;
+        st->mask |= ((1u << bits) - 1u) << shift;
+        st->k_rem = need - above;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sel_count(const void* s, int dtype, int64_t n,
+                                                   const SelState* st, int64_t* cgt,
+                                                   int64_t* ceq) {
+    const uint32_t T = st->prefix;
+    int64_t base = (int64_t)blockIdx.x * SEL_TILE;
+    int gt = 0, eq = 0;
+    for (int j = threadIdx.x; j < SEL_TILE; j += 256) {
+        int64_t i = base + j;
+        if (i < n) {
+            uint32_t key = load_key(s, dtype, i);
+            gt += key > T;
+            eq += key == T;
+        }
+    }
+    __shared__ int rg[256], re[256];
+    rg[threadIdx.x] = gt;
+    re[threadIdx.x] = eq;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            rg[threadIdx.x] += rg[threadIdx.x + o];
+            re[threadIdx.x] += re[threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        cgt[blockIdx.x] = rg[0];
+        ceq[blockIdx.x] = re[0];
+    }
+}
+
+// Exclusive scan of two arrays of nb counts, one block.
+__global__ __launch_bounds__(1024) void k_sel_scan(int64_t* a, int64_t* b, int64_t nb) {
+    __shared__ int64_t pa[1024], pb[1024];
+    int64_t per = (nb + 1023) / 1024;
+    int64_t lo = threadIdx.x * per, hi = min(nb, lo + per);
+    int64_t sa = 0, sb = 0;
+    for (int64_t i = lo; i < hi; ++i) {
+        sa += a[i];
+        sb += b[i];
+    }
+    pa[threadIdx.x] = sa;
+    pb[threadIdx.x] = sb;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int64_t ta = threadIdx.x >= off ? pa[threadIdx.x - off] : 0;
+        int64_t tb = threadIdx.x >= off ? pb[threadIdx.x - off] : 0;
+        __syncthreads();
+        pa[threadIdx.x] += ta;
+        pb[threadIdx.x] += tb;
+        __syncthreads();
+    }
+    int64_t ra = pa[threadIdx.x] - sa, rb = pb[threadIdx.x] - sb;
+    for (int64_t i = lo; i < hi; ++i) {
+        int64_t xa = a[i], xb = b[i];
+        a[i] = ra;
+        b[i] = rb;
+        ra += xa;
+        rb += xb;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_sel_write(const void* s, int dtype, int64_t n,
+                                                   const SelState* st, const int64_t* bgt,
+                                                   const int64_t* beq, int64_t k, int64_t* out) {
+    const uint32_t T = st->prefix;
+    const int64_t keq = st->k_rem;
+    int64_t gt_run = bgt[blockIdx.x], eq_run = beq[blockIdx.x];
+    const int64_t base = (int64_t)blockIdx.x * SEL_TILE;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __shared__ int wg[4], we[4];
+    for (int j0 = 0; j0 < SEL_TILE; j0 += 256) {
+        int64_t i = base + j0 + threadIdx.x;
+        uint32_t key = i < n ? load_key(s, dtype, i) : 0u;
+        bool g = i < n && key > T;
+        bool e = i < n && key == T;
+        uint64_t mg = __ballot(g), me = __ballot(e);
+        uint64_t below = (1ull << lane) - 1ull;
+        int lg = __popcll(mg & below), le = __popcll(me & below);
+        if (lane == 0) {
+            wg[wid] = __popcll(mg);
+            we[wid] = __popcll(me);
+        }
+        __syncthreads();
+        int og = 0, oe = 0;
+        for (int w = 0; w < wid; ++w) {
+            og += wg[w];
+            oe += we[w];
+        }
+        int tg = wg[0] + wg[1] + wg[2] + wg[3];
+        int te = we[0] + we[1] + we[2] + we[3];
+        int64_t gb = gt_run + og + lg;  // keys > T before i
+        int64_t eb = eq_run + oe + le;  // keys == T before i
+        const int64_t slot = gb + min(eb, keq);
+        if ((g || (e && eb < keq)) && slot < k) out[slot] = i;
+        gt_run += tg;
+        eq_run += te;
+        __syncthreads();
+    }
+}
+
+// Sort the k selected indices by (key desc, index asc); k <= SEL_SORT_MAX.
+__global__ __launch_bounds__(1024) void k_sel_sort(const void* s, int dtype, const int64_t* sel,
+                                                   int64_t k, int P, int64_t* out) {
+    extern __shared__ uint64_t comp[];
+    for (int i = threadIdx.x; i < P; i += 1024) {
+        if (i < k) {
+            int64_t idx = sel[i];
+            uint32_t key = load_key(s, dtype, idx);
+            comp[i] = ((uint64_t)(~key) << 32) | (uint64_t)(uint32_t)idx;
+        } else {
+            comp[i] = ~0ull;
+        }
+    }
+    __syncthreads();
+    for (int size = 2; size <= P; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = threadIdx.x; i < P; i += 1024) {
+                int j = i ^ stride;
+                if (j > i) {
+                    bool up = (i & size) == 0;
+                    uint64_t a = comp[i], b = comp[j];
+                    if ((a > b) == up) {
+                        comp[i] = b;
+                        comp[j] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < k; i += 1024) out[i] = (int64_t)(uint32_t)comp[i];
+}
+
+}  // namespace sseqs
+
+using namespace sseqs;
+
+extern "C" size_t sseqs_topk_ws_bytes(int64_t n) {
+    int64_t nb = ceil_div(n > 0 ? n : 1, SEL_TILE);
+    return (size_t)SEL_BINS * 4 + 64 + (size_t)nb * 16 + (size_t)SEL_SORT_MAX * 8 + 256;
+}
+
+extern "C" int sseqs_topk(const void* scores, int dtype, int64_t n, int64_t k, int sorted,
+                          int64_t* out_idx, void* ws, size_t ws_bytes, hipStream_t stream) {
+    if (n < 0 || k < 0) return fail(-1, "topk: negative size");
+    if (k > n) return fail(-1, "topk: k larger than the number of scores");
+    if (dtype != SSEQS_F32 && dtype != SSEQS_F16 && dtype != SSEQS_I32)
+        return fail(-1, "topk: unsupported dtype");
+    if (k == 0) return 0;
+    if (sorted && k > SEL_SORT_MAX) return fail(-1, "topk: sorted output limited to k <= 16384");
+    if (sorted && n > 0xffffffffLL) return fail(-1, "topk: sorted output needs n < 2^32");
+    if (ws_bytes < sseqs_topk_ws_bytes(n)) return fail(-2, "topk: workspace too small");
+    uint8_t* w = static_cast<uint8_t*>(ws);
+    uint32_t* hist = reinterpret_cast<uint32_t*>(w);
+    SelState* st = reinterpret_cast<SelState*>(w + SEL_BINS * 4);
+    const int64_t nb = ceil_div(n, SEL_TILE);
+    int64_t* cgt = reinterpret_cast<int64_t*>(w + SEL_BINS * 4 + 64);
+    int64_t* ceq = cgt + nb;
+    int64_t* tmp = ceq + nb;  // SEL_SORT_MAX entries for the sorted path
+
+    hipLaunchKernelGGL(k_sel_init, dim3(1), dim3(256), 0, stream, hist, st, (uint32_t)k);
+    if (int r = check_launch("topk init")) return r;
+    const int shifts[3] = {21, 10, 0}, bits[3] = {11, 11, 10};
+    int64_t hb = ceil_div(n, 256 * 16);
+    unsigned hgrid = (unsigned)(hb < 4096 ? (hb > 0 ? hb : 1) : 4096);
+    for (int p = 0; p < 3; ++p) {
+        hipLaunchKernelGGL(k_sel_hist, dim3(hgrid), dim3(256), 0, stream, scores, dtype, n, st, hist,
+                           shifts[p], bits[p]);
+        if (int r = check_launch("topk hist")) return r;
+        hipLaunchKernelGGL(k_sel_pick, dim3(1), dim3(1024), 0, stream, st, hist, shifts[p], bits[p]);
+        if (int r = check_launch("topk pick")) return r;
+    }
+    hipLaunchKernelGGL(k_sel_count, dim3((unsigned)nb), dim3(256), 0, stream, scores, dtype, n, st,
+                       cgt, ceq);
+    if (int r = check_launch("topk count")) return r;
+    hipLaunchKernelGGL(k_sel_scan, dim3(1), dim3(1024), 0, stream, cgt, ceq, nb);
+    if (int r = check_launch("topk scan")) return r;
+    int64_t* dst = sorted ? tmp : out_idx;
+    hipLaunchKernelGGL(k_sel_write, dim3((unsigned)nb), dim3(256), 0, stream, scores, dtype, n, st,
+                       cgt, ceq, k, dst);
+    if (int r = check_launch("topk write")) return r;
+    if (sorted) {
+        int P = 1;
+        while (P < k) P <<= 1;
+        size_t lds = (size_t)P * 8;
+        if (lds > 64 * 1024 &&
+            hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sel_sort),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+            return fail(-3, "topk: cannot enable large LDS for the sort");
+        hipLaunchKernelGGL(k_sel_sort, dim3(1), dim3(1024), lds, stream, scores, dtype, tmp, k, P,
+                           out_idx);
+        if (int r = check_launch("topk sort")) return r;
+    }
+    return 0;
+}

@@ -1,0 +1,240 @@
+// Smith-Waterman scoring with the reference's 32-stripe rule.
+//
+// Semantics (sseqs/sw.cu:55-325): rows are target residues, columns query
+// residues; the query is cut into 32 stripes of TPT = ceil(Lq/32) columns and
+// the horizontal (left) operand is 0 at the first column of every stripe,
+// while the diagonal operand crosses stripes.  Linear:
+//   H = max(0, Hdiag+s, Hup+g, Hleft+g)
+// Affine (gaps open from M only):
+//   M = max(0, s + max(M,X,Y)diag), X = max(0, Mleft-o, Xleft-e),
+//   Y = max(0, Mup-o, Yup-e), score = max over cells of max(M,X,Y).
+//
+// Layout on gfx950: one wave64 scores two targets, lanes 0-31 and 32-63 each
+// own the 32 stripes of one target and sweep its rows; per row a lane reads
+// its stripe of the query profile (int8, [residue][lane][TPT4]) from LDS and
+// takes the diagonal operand of its first column from the left lane with one
+// cross-lane shift.  Both halves run max(T0, T1) rows; the shorter target is
+// padded with a non-residue row (score -4 everywhere), which can never raise
+// the running maximum, so scores are unchanged.  Columns past the query end
+// are padded the same way.
+#include "common.h"
+
+namespace sseqs {
+
+constexpr int SW_CH = 256;  // target rows staged per LDS refill
+
+// Targets given as the reference passes them: good_idx into a delimiter
+// position array P over an ASCII batch (sseqs/sw.cu:67-77).
+struct SrcAscii {
+    const uint8_t* ascii;
+    const int32_t* P;
+    const int32_t* good_idx;
+    __device__ bool target(int64_t g, int64_t& start, int& len) const {
+        int gi = good_idx[g];
+        int64_t prevp = gi > 0 ? (int64_t)P[gi - 1] : -1;  // P[-1] taken as -1
+        int64_t st = prevp + 1, stop = (int64_t)P[gi] + 1;
+        int64_t length = stop - st - 1;
+        start = st;
+        len = length > 0 ? (int)length : 0;
+        return !(length == 0 || stop < st);
+    }
+    __device__ uint32_t sym(int64_t start, int p) const { return ascii[start + p]; }
+};
+
+// Targets given as candidate ids of the HBM-resident packed database.
+struct SrcPacked {
+    const uint8_t* packed;
+    const int64_t* off;
+    const int32_t* len;
+    const int64_t* cand;
+    __device__ bool target(int64_t g, int64_t& start, int& l) const {
+        int64_t s = cand[g];
+        start = off[s];
+        l = len[s];
+        return l > 0;
+    }
+    __device__ uint32_t sym(int64_t start, int p) const { return packed_sym(packed, start + p); }
+};
+
+template <int TPT, bool AFFINE, class Src>
+__global__ __launch_bounds__(64) void k_sw(Src src, const uint8_t* __restrict__ query, int Lq,
+                                           int64_t n, int p1, int p2, int32_t* __restrict__ out,
+                                           Blosum bl) {
+    constexpr int TPT4 = (TPT + 3) & ~3;
+    constexpr int NW = TPT4 / 4;
+    __shared__ int8_t s_bl[400];
+    __shared__ uint32_t s_prof[21 * 32 * NW];
+    __shared__ uint8_t s_res[2][SW_CH];
+
+    const int tid = threadIdx.x;
+    const int half = tid >> 5, lane = tid & 31;
+
+    for (int i = tid; i < 400; i += 64) s_bl[i] = bl.v[i];
+    __syncthreads();
+    // profile word w of (residue c, lane ln) packs columns ln*TPT + 4w .. +3
+    for (int idx = tid; idx < 21 * 32 * NW; idx += 64) {
+        int c = idx / (32 * NW);
+        int rem = idx - c * 32 * NW;
+        int ln = rem / NW, w = rem - ln * NW;
+        uint32_t word = 0;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            int k = 4 * w + b;
+            int j = ln * TPT + k;
+            int v = -4;
+            if (k < TPT && j < Lq && c < 20) {
+                int qc = query[j];
+                v = qc < 20 ? (int)s_bl[c * 20 + qc] : -4;
+            }
+            word |= (uint32_t)(uint8_t)(int8_t)v << (8 * b);
+        }
+        s_prof[idx] = word;
+    }
+    __syncthreads();
+
+    const int go = p1, ge = p2;  // affine: open, extend; linear: p1 = gap (<= 0)
+    for (int64_t pair = blockIdx.x; pair * 2 < n; pair += gridDim.x) {
+        const int64_t g = pair * 2 + half;
+        int64_t start = 0;
+        int len = 0;
+        bool valid = false;
+        if (g < n) valid = src.target(g, start, len);
+        const int T = valid ? len : 0;
+        const int Tmax = max(T, __shfl_xor(T, 32, 64));
+
+        int Hp[TPT], Mp[TPT], Yp[TPT];
+#pragma unroll
+        for (int k = 0; k < TPT; ++k) { Hp[k] = 0; Mp[k] = 0; Yp[k] = 0; }
+        int best = 0;
+
+        for (int c0 = 0; c0 < Tmax; c0 += SW_CH) {
+            const int rows = min(SW_CH, Tmax - c0);
+            __syncthreads();
+            for (int r = lane; r < rows; r += 32) {
+                int p = c0 + r;
+                s_res[half][r] = (uint8_t)(p < T ? aa_index(src.sym(start, p)) : 20);
+            }
+            __syncthreads();
+            for (int r = 0; r < rows; ++r) {
+                const int c = s_res[half][r];
+                const uint32_t* pr = &s_prof[(c * 32 + lane) * NW];
+                uint32_t pw[NW];
+#pragma unroll
+                for (int w = 0; w < NW; ++w) pw[w] = pr[w];
+                int din = __shfl_up(Hp[TPT - 1], 1, 32);
+                if (lane == 0) din = 0;
+                int dg = din;
+                if constexpr (AFFINE) {
+                    int lM = 0, lX = 0;
+#pragma unroll
+                    for (int k = 0; k < TPT; ++k) {
+                        const int s = (int)(int8_t)(pw[k >> 2] >> (8 * (k & 3)));
+                        const int m = max(dg + s, 0);
+                        const int x = max(max(lM - go, lX - ge), 0);
+                        const int y = max(max(Mp[k] - go, Yp[k] - ge), 0);
+                        const int h = max(m, max(x, y));
+                        dg = Hp[k];
+                        Hp[k] = h;
+                        Mp[k] = m;
+                        Yp[k] = y;
+                        lM = m;
+                        lX = x;
+                        best = max(best, h);
+                    }
+                } else {
+                    int left = 0;
+#pragma unroll
+                    for (int k = 0; k < TPT; ++k) {
+                        const int s = (int)(int8_t)(pw[k >> 2] >> (8 * (k & 3)));
+                        const int h = max(max(dg + s, Hp[k] + go), max(left + go, 0));
+                        dg = Hp[k];
+                        Hp[k] = h;
+                        left = h;
+                        best = max(best, h);
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int off = 16; off > 0; off >>= 1) best = max(best, __shfl_xor(best, off, 32));
+        if (lane == 0 && g < n) out[g] = valid ? best : -1;
+    }
+}
+
+template <bool AFFINE, class Src>
+static int launch_sw(const Src& src, const uint8_t* q, int Lq, int64_t n, int p1, int p2,
+                     int32_t* out, hipStream_t stream) {
+    const Blosum bl = blosum62();
+    int64_t pairs = ceil_div(n, 2);
+    unsigned grid = (unsigned)(pairs < 16384 ? pairs : 16384);
+    const int tpt = (Lq + 31) >> 5;
+    switch (tpt) {
+#define SW_CASE(T)                                                                              \
+    case T:                                                                                     \
+        hipLaunchKernelGGL((k_sw<T, AFFINE, Src>), dim3(grid), dim3(64), 0, stream, src, q, Lq, \
+                           n, p1, p2, out, bl);                                                 \
+        break;
+        SW_CASE(1) SW_CASE(2) SW_CASE(3) SW_CASE(4) SW_CASE(5) SW_CASE(6) SW_CASE(7) SW_CASE(8)
+        SW_CASE(9) SW_CASE(10) SW_CASE(11) SW_CASE(12) SW_CASE(13) SW_CASE(14) SW_CASE(15)
+        SW_CASE(16) SW_CASE(17) SW_CASE(18) SW_CASE(19) SW_CASE(20) SW_CASE(21) SW_CASE(22)
+        SW_CASE(23) SW_CASE(24) SW_CASE(25) SW_CASE(26) SW_CASE(27) SW_CASE(28) SW_CASE(29)
+        SW_CASE(30) SW_CASE(31) SW_CASE(32)
+#undef SW_CASE
+    default:
+        return fail(-1, "sw: query length must be 1..1024");
+    }
+    return check_launch(AFFINE ? "sw affine" : "sw linear");
+}
+
+__global__ void k_fill_i32(int32_t* p, int64_t n, int32_t v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+static int fill_minus_one(int32_t* out, int64_t n, hipStream_t stream) {
+    hipLaunchKernelGGL(k_fill_i32, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, out, n,
+                       -1);
+    return check_launch("sw fill");
+}
+
+}  // namespace sseqs
+
+using namespace sseqs;
+
+extern "C" int launch_sw_cuda_blosum62(const uint8_t* q, int Lq, const int32_t* good_idx,
+                                       const uint8_t* ascii, const int32_t* P, int n, int32_t* out,
+                                       int gap_val, hipStream_t stream) {
+    if (n < 0) return fail(-1, "sw_cuda_blosum62: negative count");
+    if (n == 0) return 0;
+    if (Lq < 0 || Lq > 1024) return fail(-1, "sw_cuda_blosum62: query length must be 0..1024");
+    if (gap_val > 0) return fail(-1, "sw_cuda_blosum62: gap_penalty must be <= 0");
+    if (Lq == 0) return fill_minus_one(out, n, stream);
+    SrcAscii src{ascii, P, good_idx};
+    return launch_sw<false>(src, q, Lq, n, gap_val, 0, out, stream);
+}
+
+extern "C" int launch_sw_cuda_affine(const uint8_t* q, int Lq, const int32_t* good_idx,
+                                     const uint8_t* ascii, const int32_t* P, int n, int32_t* out,
+                                     int gap_open, int gap_extend, hipStream_t stream) {
+    if (n < 0) return fail(-1, "sw_cuda_affine: negative count");
+    if (n == 0) return 0;
+    if (Lq < 0 || Lq > 1024) return fail(-1, "sw_cuda_affine: query length must be 0..1024");
+    if (gap_open < 0 || gap_extend < 0)
+        return fail(-1, "sw_cuda_affine: gap_open and gap_extend must be >= 0");
+    if (Lq == 0) return fill_minus_one(out, n, stream);
+    SrcAscii src{ascii, P, good_idx};
+    return launch_sw<true>(src, q, Lq, n, gap_open, gap_extend, out, stream);
+}
+
+extern "C" int sseqs_db_sw_affine(const uint8_t* packed, const int64_t* seq_off,
+                                  const int32_t* seq_len, const int64_t* cand, int64_t n,
+                                  const uint8_t* query, int Lq, int gap_open, int gap_extend,
+                                  int32_t* out, hipStream_t stream) {
+    if (n < 0) return fail(-1, "db_sw_affine: negative count");
+    if (n == 0) return 0;
+    if (Lq < 1 || Lq > 1024) return fail(-1, "db_sw_affine: query length must be 1..1024");
+    if (gap_open < 0 || gap_extend < 0)
+        return fail(-1, "db_sw_affine: gap_open and gap_extend must be >= 0");
+    SrcPacked src{packed, seq_off, seq_len, cand};
+    return launch_sw<true>(src, query, Lq, n, gap_open, gap_extend, out, stream);
+}

@@ -1,0 +1,189 @@
+"""The HBM-resident sequence database.
+
+The reference keeps its ``.xbit`` database in pinned host RAM and streams it
+over PCIe, 1 GB chunk by chunk, for every search (sseqs/msa.py:13-64,
+196-208), re-finding the delimiters of every batch on the GPU
+(sseqs/msa.py:227-267).  Here the packed bytes are copied to HBM once and an
+index of every sequence (symbol offset, length) is built once by the
+delimiter-scan kernel.  The index is kept in ascending length order (a
+permutation of the file order when the file is not already sorted) so that
+the prefilter can run one correlation size per contiguous range and the
+MFMA tiles hold sequences of similar length.
+"""
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call, ptr, stream_of
+
+PAD = 16  # tail bytes so that 2-byte symbol reads never leave the buffer
+
+
+@dataclass
+class SeqDB:
+    packed: torch.Tensor            # uint8, 5-bit stream + PAD bytes
+    n_syms: int                     # symbols in the stream
+    seq_off: torch.Tensor           # int64 [n], symbol offset, length order
+    seq_len: torch.Tensor           # int32 [n], length order
+    orig_id: torch.Tensor | None    # int64 [n] file-order id, None when identity
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def n(self) -> int:
+        return int(self.seq_len.numel())
+
+    @property
+    def device(self) -> torch.device:
+        return self.packed.device
+
+    @property
+    def nbytes(self) -> int:
+        """Packed bytes of the database (what one sweep streams)."""
+        return (self.n_syms * 5 + 7) // 8
+
+    @property
+    def residues(self) -> int:
+        return int(self.extra.get("residues", 0)) or int(self.seq_len.sum().item())
+
+    # ------------------------------------------------------------ ranges --
+    def length_ranges(self, Lq: int, nmax: int = 2048) -> list[tuple[int, int, int]]:
+        """[(N, begin, end)] with one correlation size N = nextpow2(max(Lq, L))
+        per contiguous range of the length-sorted index (sequences longer
+        than nmax share N = nmax and are truncated to it)."""
+        key = ("ranges", Lq, nmax)
+        if key in self.extra:
+            return self.extra[key]
+        bounds = []
+        N = 16
+        while N < Lq:
+            N <<= 1
+        lens = self.seq_len
+        begin = 0
+        while begin < self.n:
+            Ncur = min(N, nmax)
+            if Ncur < nmax:
+                end = int(torch.searchsorted(lens, torch.tensor(Ncur, dtype=lens.dtype,
+                                                                 device=lens.device),
+                                             right=True).item())
+            else:
+                end = self.n
+            if end > begin:
+                bounds.append((Ncur, begin, end))
+                begin = end
+            N <<= 1
+        self.extra[key] = bounds
+        return bounds
+
+    # ----------------------------------------------------------- gathers --
+    def gather(self, ids: torch.Tensor, width: int = 500) -> tuple[torch.Tensor, torch.Tensor]:
+        """ASCII rows (n, width) and lengths of sequences ``ids`` (index order),
+        truncated to ``width`` (the reference keeps at most 500 residues per
+        hit, sseqs/msa.py:183,367)."""
+        ids = ids.to(torch.int64).contiguous()
+        n = ids.numel()
+        out = torch.zeros((n, width), dtype=torch.uint8, device=self.device)
+        ln = torch.empty(n, dtype=torch.int32, device=self.device)
+        if n:
+            call("sseqs_db_gather", ptr(self.packed), ptr(self.seq_off), ptr(self.seq_len),
+                 ptr(ids), n, int(width), ptr(out), ptr(ln), stream_of(self.device))
+        return out, ln
+
+    def strings(self, ids: torch.Tensor, width: int = 500) -> list[str]:
+        rows, ln = self.gather(ids, width)
+        rows, ln = rows.cpu().numpy(), ln.cpu().numpy()
+        return [rows[i, : ln[i]].tobytes().decode("latin-1") for i in range(rows.shape[0])]
+
+    def file_ids(self, ids: torch.Tensor) -> torch.Tensor:
+        return ids if self.orig_id is None else self.orig_id[ids]
+
+    # ------------------------------------------------------ constructors --
+    @staticmethod
+    def from_packed(packed: torch.Tensor, n_syms: int | None = None) -> "SeqDB":
+        """Index a packed stream already in HBM (every non-'@' run is one
+        sequence; empty runs are skipped like sseqs/msa.py:253)."""
+        dev = packed.device
+        nbytes = packed.numel()
+        if n_syms is None:
+            n_syms = nbytes * 8 // 5
+        if (n_syms * 5 + 7) // 8 + PAD > nbytes:
+            buf = torch.zeros((n_syms * 5 + 7) // 8 + PAD, dtype=torch.uint8, device=dev)
+            buf[: min(nbytes, buf.numel())] = packed[: buf.numel()]
+            packed = buf
+        lib = _lib.load()
+        ws = _lib.workspace(lib.sseqs_db_index_ws_bytes(n_syms), dev)
+        cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+        # pass 1 with capacity 0 only counts; pass 2 writes
+        dummy = torch.empty(1, dtype=torch.int64, device=dev)
+        call("sseqs_db_index", ptr(packed), n_syms, ptr(dummy), ptr(dummy), 0, ptr(cnt), ptr(ws),
+             ws.numel(), stream_of(dev))
+        n = int(cnt.item())
+        off = torch.empty(n, dtype=torch.int64, device=dev)
+        ln = torch.empty(n, dtype=torch.int32, device=dev)
+        call("sseqs_db_index", ptr(packed), n_syms, ptr(off), ptr(ln), n, ptr(cnt), ptr(ws),
+             ws.numel(), stream_of(dev))
+        orig = None
+        if n > 1 and bool((ln[1:] < ln[:-1]).any().item()):
+            orig = torch.sort(ln, stable=True).indices
+            off, ln = off[orig].contiguous(), ln[orig].contiguous()
+        return SeqDB(packed, int(n_syms), off, ln, orig)
+
+    @staticmethod
+    def from_xbit(path: str | os.PathLike, chunks: int | None = None, device="cuda") -> "SeqDB":
+        """Load an .xbit file (or its first ``chunks`` x 1e9 bytes, like the
+        reference's CHUNKS, sseqs/msa.py:14,50-55) into HBM and index it."""
+        dev = torch.device(device)
+        size = os.path.getsize(path)
+        if chunks is not None:
+            size = min(size, int(chunks) * 1_000_000_000)
+        packed = torch.zeros(size + PAD, dtype=torch.uint8, device=dev)
+        step = 1 << 28
+        host = torch.empty(step, dtype=torch.uint8, pin_memory=True)
+        with open(path, "rb") as fh:
+            pos = 0
+            while pos < size:
+                m = min(step, size - pos)
+                got = fh.readinto(host[:m].numpy())
+                if got != m:
+                    raise IOError(f"short read from {path}")
+                packed[pos: pos + m].copy_(host[:m], non_blocking=False)
+                pos += m
+        return SeqDB.from_packed(packed, size * 8 // 5)
+
+    @staticmethod
+    def synthetic(n: int, queries: list[str], seed: int = 0, device="cuda",
+                  hom_period: int = 20) -> "SeqDB":
+        """The benchmark's synthetic database (sseqs_amd.synth), generated
+        directly in HBM, already in length order."""
+        from .synth import generate_db
+        g = generate_db(n, queries, seed=seed, device=device, hom_period=hom_period)
+        cls = g.classes
+        residues = int((cls.count * cls.length.astype(np.int64)).sum())
+        return SeqDB(g.packed, g.n_syms, g.seq_off, g.seq_len, None,
+                     extra={"hom_query": g.hom_query, "classes": cls, "residues": residues})
+
+
+_default_db: SeqDB | None = None
+
+
+def default_db(device="cuda") -> SeqDB:
+    """Database named by DBPATH (default uniref_bfd_mgy_cf.xbit), first CHUNKS
+    GB (sseqs/msa.py:14,18), loaded on first use."""
+    global _default_db
+    if _default_db is None:
+        path = os.environ.get("DBPATH", "uniref_bfd_mgy_cf.xbit")
+        if not os.path.exists(path):
+            raise FileNotFoundError(
+                f"Didn't find MSA database at `{path}` (set DBPATH, or pass db=...)")
+        chunks = os.environ.get("CHUNKS")
+        _default_db = SeqDB.from_xbit(path, int(chunks) if chunks else None, device=device)
+    return _default_db
+
+
+def set_default_db(db: SeqDB | None) -> None:
+    global _default_db
+    _default_db = db

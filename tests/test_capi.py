@@ -1,0 +1,49 @@
+"""The C-ABI library loads without a GPU and exports exactly what
+include/sseqs_amd.h declares (no compute calls here)."""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def header_functions():
+    text = (ROOT / "include" / "sseqs_amd.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][\w\s\*]*?\b([a-z_][a-z0-9_]*)\s*\(", text, flags=re.M)
+    return sorted(set(n for n in names if n not in ("if", "while", "return", "sizeof")))
+
+
+def test_header_parses():
+    names = header_functions()
+    assert "launch_sw_cuda_affine" in names and "sseqs_topk" in names
+    assert len(names) >= 15
+
+
+def test_library_exports_every_symbol():
+    from sseqs_amd import _lib
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("library not built")
+    lib = ctypes.CDLL(str(_lib.LIB_PATH))
+    missing = [n for n in header_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    # the Python binding table covers the same set
+    assert sorted(_lib.SIGNATURES) == header_functions()
+    assert lib.sseqs_abi_version() == 1
+
+
+def test_argument_errors_need_no_gpu():
+    """Host-side argument checks return before any launch."""
+    from sseqs_amd import _lib
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("library not built")
+    lib = _lib.load()
+    assert lib.launch_sw_cuda_affine(None, 2000, None, None, None, 5, None, 11, 1, None) == -1
+    assert b"query length" in lib.sseqs_last_error()
+    assert lib.launch_sw_cuda_blosum62(None, 10, None, None, None, 5, None, 3, None) == -1
+    assert lib.sseqs_topk(None, 0, 10, 11, 1, None, None, 0, None) == -1
+    assert lib.launch_warp_corr(None, None, None, None, None, None, None, None, 4, 21, 4, 10, 10,
+                                100, None, None) == -1
+    assert b"power of two" in lib.sseqs_last_error()

@@ -1,0 +1,272 @@
+"""GPU parity of every kernel against the CPU oracle, through the C ABI.
+
+Integer work (codec, index, SW, top-k, traceback, generator) must be
+bit-exact.  The prefilter is floating point: the tolerance, against the
+FP64 closed form, is |x - x64| <= 1e-2 * |x64| + 1e-3 * max|x64| (the
+second term covers scores near zero; fp16 outputs add at most 2^-11
+relative).
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+AA = "ARNDCQEGHILKMFPSTWYV"
+
+
+def rand_seq(rng, n, alphabet=AA):
+    return "".join(rng.choice(list(alphabet), int(n)))
+
+
+def mutate(rng, s, sub=0.3, indel=0.03):
+    out = []
+    for c in s:
+        r = rng.random()
+        if r < indel / 2:
+            continue
+        if r < indel:
+            out.append(rand_seq(rng, rng.integers(1, 4)))
+        out.append(rand_seq(rng, 1) if rng.random() < sub else c)
+    return "".join(out) or "A"
+
+
+def corr_tol(got, want):
+    scale = 1e-3 * max(1.0, float(np.max(np.abs(want))))
+    return np.abs(got - want) <= 1e-2 * np.abs(want) + scale
+
+
+# ----------------------------------------------------------------- codec --
+
+@pytest.mark.parametrize("n", [1, 7, 8, 9, 1000, 123457])
+def test_xbit_roundtrip(orc, dev, n):
+    from sseqs_amd import ops
+    announce(f"xbit n={n}")
+    rng = np.random.default_rng(n)
+    s = (rng.integers(0, 32, n) + 64).astype(np.uint8)
+    packed = ops.compress(torch.from_numpy(s).to(dev))
+    assert np.array_equal(packed.cpu().numpy(), orc.xbit_pack(s))
+    assert np.array_equal(ops.decompress(packed, n, 0).cpu().numpy(), s)
+    for g in (1, 3):
+        if 8 * g < n:
+            out = ops.decompress(packed, n - 8 * g, 5 * g).cpu().numpy()
+            assert np.array_equal(out, s[8 * g:])
+
+
+def test_xbit_host_input_moves_to_gpu(dev):
+    from sseqs_amd import ops
+    s = torch.tensor(list(b"HELLO@WORLD@"), dtype=torch.uint8)
+    packed = ops.compress(s)
+    assert packed.is_cuda
+    assert bytes(ops.decompress(packed, 12, 0).cpu().numpy()) == b"HELLO@WORLD@"
+
+
+def _packed_db(dev, text: bytes):
+    from sseqs_amd import ops
+    t = torch.from_numpy(np.frombuffer(text, dtype=np.uint8).copy()).to(dev)
+    return ops.compress(t)
+
+
+def test_db_index_and_gather(orc, dev):
+    from sseqs_amd.db import SeqDB
+    announce("db_index")
+    rng = np.random.default_rng(2)
+    seqs = [rand_seq(rng, L) for L in rng.integers(1, 700, 3000)]
+    # leading '@', doubled '@' and no trailing '@' exercise empty runs
+    text = ("@" + "@".join(seqs[:10]) + "@@" + "@".join(seqs[10:])).encode()
+    db = SeqDB.from_packed(_packed_db(dev, text), len(text))
+    assert db.n == len(seqs)
+    lens = np.array([len(s) for s in seqs])
+    order = np.argsort(lens, kind="stable")
+    assert np.array_equal(db.seq_len.cpu().numpy(), lens[order])
+    assert np.array_equal(db.orig_id.cpu().numpy(), order)
+    ids = torch.arange(0, db.n, 7, device=dev)
+    got = db.strings(ids, 500)
+    want = [seqs[order[i]][:500] for i in ids.cpu().numpy()]
+    assert got == want
+
+
+# -------------------------------------------------------------------- SW --
+
+def _batch(rng, query, n):
+    """An ASCII batch of n targets joined by '@' with its delimiter positions."""
+    tg = []
+    for i in range(n):
+        if i % 3 == 0:
+            tg.append(mutate(rng, query[int(rng.integers(0, max(1, len(query) // 2))):]))
+        else:
+            tg.append(rand_seq(rng, rng.integers(1, 600)))
+    text = "@".join(tg) + "@"
+    a = np.frombuffer(text.encode(), dtype=np.uint8).copy()
+    D = np.nonzero(a == 64)[0].astype(np.int32)
+    return a, D
+
+
+@pytest.mark.parametrize("Lq", [1, 2, 5, 17, 31, 32, 33, 40, 63, 64, 65, 100, 200, 257, 500,
+                                777, 1000, 1024])
+def test_sw_kernels_match_oracle(orc, dev, Lq):
+    from sseqs_amd import ops
+    announce(f"sw Lq={Lq}")
+    rng = np.random.default_rng(Lq)
+    query = rand_seq(rng, Lq)
+    qc = orc.encode_query_sw(query)
+    a, D = _batch(rng, query, 150)
+    gi = np.sort(rng.choice(len(D), size=101, replace=False)).astype(np.int32)
+    gi[0] = 0  # target 0 reads P[-1]
+    starts = (D + 1)[(D + 1) < a.size].astype(np.int32)
+    tq, tgi, ta = (torch.from_numpy(x).to(dev) for x in (qc, gi, a))
+    tD, tS = torch.from_numpy(D).to(dev), torch.from_numpy(starts).to(dev)
+    for gap in (-1, -4, 0):
+        got = ops.sw_cuda_blosum62(tq, tgi, ta, tD, gap_penalty_param=gap).cpu().numpy()
+        assert np.array_equal(got, orc.sw_linear(qc, gi, a, D, gap)), gap
+    for o, e in ((11, 1), (5, 2), (0, 0)):
+        got = ops.sw_cuda_affine(tq, tgi, ta, tD, o, e).cpu().numpy()
+        assert np.array_equal(got, orc.sw_affine(qc, gi, a, D, o, e)), (o, e)
+    # the affine stage of the cascade passes starts = D + 1 (sseqs/msa.py:331)
+    gi2 = gi[gi < starts.size]
+    got = ops.sw_cuda_affine(tq, torch.from_numpy(gi2).to(dev), ta, tS).cpu().numpy()
+    assert np.array_equal(got, orc.sw_affine(qc, gi2, a, starts))
+
+
+def test_sw_edge_cases(orc, dev):
+    from sseqs_amd import ops
+    q = orc.encode_query_sw("WCWHW")
+    a = np.frombuffer(b"WCW@@HWA@W@", dtype=np.uint8).copy()
+    P = np.array([3, 4, 8, 10], np.int32)
+    gi = np.array([0, 1, 2, 3], np.int32)
+    t = lambda x: torch.from_numpy(x).to(dev)
+    got = ops.sw_cuda_affine(t(q), t(gi), t(a), t(P)).cpu().numpy()
+    assert np.array_equal(got, orc.sw_affine(q, gi, a, P))
+    assert got[1] == -1
+    empty = ops.sw_cuda_affine(t(np.zeros(0, np.uint8)), t(gi), t(a), t(P)).cpu().numpy()
+    assert empty.tolist() == [-1, -1, -1, -1]
+    with pytest.raises(RuntimeError):
+        ops.sw_cuda_affine(t(np.zeros(1025, np.uint8)), t(gi), t(a), t(P))
+    with pytest.raises(RuntimeError):
+        ops.sw_cuda_blosum62(t(q), t(gi), t(a), t(P), gap_penalty_param=2)
+    with pytest.raises(TypeError):
+        ops.sw_cuda_affine(t(q), t(gi.astype(np.int64)), t(a), t(P))
+
+
+def test_public_sw(orc, dev):
+    import sseqs_amd
+    rng = np.random.default_rng(9)
+    q = rand_seq(rng, 120)
+    targets = [mutate(rng, q) for _ in range(5)] + [rand_seq(rng, 80) for _ in range(5)]
+    got = sseqs_amd.sw(q, targets).cpu().numpy()
+    qc = orc.encode_query_sw(q)
+    assert got.tolist() == [orc.sw_affine_one(qc, t) for t in targets]
+
+
+# ------------------------------------------------------------- prefilter --
+
+@pytest.mark.parametrize("N,k", [(16, 1), (64, 4), (128, 2), (256, 4), (512, 4), (1024, 4),
+                                 (2048, 4), (512, 8), (256, 20)])
+def test_warp_corr_matches_fp64(orc, dev, N, k):
+    from sseqs_amd import ops
+    announce(f"warp_corr N={N} k={k}")
+    rng = np.random.default_rng(N * 31 + k)
+    Lq = int(rng.integers(max(2, N // 2), N + 1))
+    query = rand_seq(rng, Lq, AA + "X")
+    A, B = orc.weighted_luts(query, k)
+    qrev = orc.encode_query_rev(query)
+    segs = [mutate(rng, query) if i % 4 == 0 else rand_seq(rng, rng.integers(1, N + 1), AA + "XB")
+            for i in range(97)]
+    segs = [s[:N] for s in segs]
+    flat = ("@".join(segs) + "@").encode()
+    lens = np.array([len(s) for s in segs], np.int32)
+    off = np.concatenate([[0], np.cumsum(lens + 1)[:-1]]).astype(np.int32)
+    maxlen = int(lens.max()) if k != 8 else int(lens.max()) // 2  # exercise the clamp
+    want = orc.corr(qrev, A, B, k, flat, off, lens, maxlen, N)
+    t = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)
+    scores = torch.zeros(len(segs), dtype=torch.float16, device=dev)
+    ops.warp_corr(scores, t(qrev), t(A), t(B), t(np.frombuffer(flat, np.uint8).copy()), t(off),
+                  t(lens), k, Lq, maxlen, N, torch.zeros(1, dtype=torch.float16, device=dev))
+    got = scores.float().cpu().numpy()
+    ok = corr_tol(got, want)
+    assert ok.all(), (N, k, np.max(np.abs(got - want)), got[~ok][:5], want[~ok][:5])
+
+
+def test_warp_corr_float32_corr_is_not_updated(dev):
+    """A float32 corr goes to a temporary fp16 copy (sseqs/filter_bind.cpp:34-35)."""
+    from sseqs_amd import ops
+    from sseqs_amd.query import prepare
+    q = prepare("ACDEFGHIKLMNPQRSTVWY" * 3)
+    flat = torch.tensor(list(b"ACDEF@GHIKL@"), dtype=torch.uint8, device=dev)
+    c = torch.zeros(2, dtype=torch.float32, device=dev)
+    ops.warp_corr(c, q.qrev, q.lut_a, q.lut_b, flat, torch.tensor([0, 6], dtype=torch.int32,
+                  device=dev), torch.tensor([5, 5], dtype=torch.int32, device=dev), 4, 60, 5,
+                  64, torch.zeros(1, dtype=torch.float16, device=dev))
+    assert c.abs().sum().item() == 0.0
+
+
+# ---------------------------------------------------------------- top-k --
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float16, torch.int32])
+@pytest.mark.parametrize("n,k", [(1, 1), (10, 3), (5000, 100), (300000, 4096), (300000, 77777)])
+def test_topk(orc, dev, dtype, n, k):
+    from sseqs_amd import ops
+    announce(f"topk {dtype} n={n} k={k}")
+    rng = np.random.default_rng(n + k)
+    if dtype == torch.int32:
+        x = rng.integers(-50, 50, n).astype(np.int32)  # heavy ties
+    else:
+        x = np.round(rng.standard_normal(n), 2).astype(np.float32)
+        x[rng.integers(0, n, max(1, n // 100))] = -np.inf
+        if dtype == torch.float16:
+            x = x.astype(np.float16)
+    t = torch.from_numpy(x).to(dev)
+    want = orc.topk_desc(x.astype(np.float64), k)
+    if k <= 16384:
+        got = ops.topk_indices(t, k, sorted=True).cpu().numpy()
+        assert np.array_equal(got, want)
+    got_u = ops.topk_indices(t, k, sorted=False).cpu().numpy()
+    assert np.array_equal(got_u, np.sort(want))
+
+
+def test_topk_errors(dev):
+    from sseqs_amd import ops
+    t = torch.zeros(10, device=dev)
+    with pytest.raises(ValueError):
+        ops.topk_indices(t, 11)
+    assert ops.topk_indices(t, 0).numel() == 0
+
+
+# ------------------------------------------------------------- backtrack --
+
+def test_backtrack_matches_oracle(orc, dev):
+    from sseqs_amd.sw import sw_affine_backtrack
+    announce("backtrack")
+    rng = np.random.default_rng(21)
+    for Lq in (1, 7, 64, 65, 130, 300):
+        query = rand_seq(rng, Lq, AA + "X")
+        targets = [mutate(rng, query) for _ in range(20)] + \
+                  [rand_seq(rng, rng.integers(1, 400), AA + "B") for _ in range(20)]
+        for o, e in ((11, 1), (3, 1)):
+            _, best, got = sw_affine_backtrack(query, targets, o, e)
+            want = [orc.backtrack(query, t, o, e) for t in targets]
+            assert got == want, (Lq, o, e)
+            assert best.cpu().tolist() == [int(w["score"]) for w in want]
+
+
+# ------------------------------------------------------------- generator --
+
+def test_generator_matches_cpu(orc, dev):
+    from sseqs_amd import synth
+    from sseqs_amd.db import SeqDB
+    announce("generator")
+    queries = [synth.make_query(200, 0), synth.make_query(57, 1)]
+    n = 20000
+    g = synth.generate_db(n, queries, seed=5, device=dev)
+    cls = g.classes
+    ascii_cpu, homq = orc.gen_db(synth.gen_params(5), queries, cls.start, cls.length, cls.sym, n)
+    got = g.packed[: (g.n_syms * 5 + 7) // 8].cpu().numpy()
+    assert np.array_equal(got, orc.xbit_pack(ascii_cpu))
+    assert np.array_equal(g.hom_query.cpu().numpy(), homq)
+    assert (homq >= 0).sum() == n // 20
+    # the index the generator writes equals the delimiter scan of its stream
+    db = SeqDB.from_packed(g.packed, g.n_syms)
+    assert db.orig_id is None
+    assert torch.equal(db.seq_off, g.seq_off) and torch.equal(db.seq_len, g.seq_len)
