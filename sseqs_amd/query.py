@@ -57,6 +57,7 @@ class PreparedQuery:
     lut_b_flat: torch.Tensor  # float32 [20*21 + 1]: flat LUT_B plus a 0 that code 21
                               # of the last row reads (see launch_warp_corr)
     k: int                   # prefilter rank actually used
+    qbytes: torch.Tensor     # uint8 raw query text (the backtrack scores non-residues -5)
 
     @property
     def length(self) -> int:
@@ -78,4 +79,6 @@ def prepare(query: str, fft_rank: int = 4, device="cuda") -> PreparedQuery:
         lut_b=torch.from_numpy(Bt).to(dev),
         lut_b_flat=torch.from_numpy(lut_b).to(dev),
         k=k,
+        qbytes=torch.from_numpy(np.frombuffer(query.encode("latin-1", errors="replace"),
+                                              dtype=np.uint8).copy()).to(dev),
     )

@@ -84,6 +84,80 @@ def global_search(db: SeqDB, q: PreparedQuery, top_prefilter: int = 100_000,
     return GlobalHits(cand[top], cs[top], pre, cand, cs)
 
 
+@dataclass
+class DeviceAlignments:
+    """Backtrack output of ``n`` hits, all on the device."""
+
+    out_q: torch.Tensor   # uint8 (n, A), right-aligned aligned query rows
+    out_t: torch.Tensor   # uint8 (n, A), right-aligned aligned target rows
+    info: torch.Tensor    # int32 (n, 6): len, q_start, q_end, t_start, t_end, score
+    width: int            # A
+
+
+class BacktrackWorkspace:
+    """Reusable device buffers for backtracking up to ``n_max`` hits of at
+    most ``width`` residues against queries of up to ``lq_max`` residues."""
+
+    def __init__(self, n_max: int, lq_max: int, width: int, device):
+        self.n_max, self.lq_max, self.width = n_max, lq_max, width
+        self.A = lq_max + width + 1
+        dev = torch.device(device)
+        self.rows = torch.zeros((n_max, width), dtype=torch.uint8, device=dev)
+        self.lens = torch.zeros(n_max, dtype=torch.int32, device=dev)
+        self.t_off = torch.arange(n_max, dtype=torch.int64, device=dev) * width
+        self.out_q = torch.zeros((n_max, self.A), dtype=torch.uint8, device=dev)
+        self.out_t = torch.zeros((n_max, self.A), dtype=torch.uint8, device=dev)
+        self.info = torch.zeros((n_max, 6), dtype=torch.int32, device=dev)
+        total = lq_max * n_max * width
+        nbytes = _lib.load().sseqs_backtrack_ws_bytes(lq_max, total, n_max)
+        self.ws = _lib.workspace(nbytes, dev)
+
+
+def backtrack_hits(db: SeqDB, q: PreparedQuery, ids: torch.Tensor,
+                   ws: BacktrackWorkspace, gap_open: int = 11,
+                   gap_extend: int = 1) -> DeviceAlignments:
+    """Gather hits ``ids`` (truncated to ``ws.width`` residues, as the
+    reference truncates a3m hits to 500) and align them to the query on the
+    device, without a host round trip.  The targets sit at a fixed stride of
+    ``width`` so no host-side offsets are needed."""
+    n = int(ids.numel())
+    Lq = len(q.text)
+    if n > ws.n_max or Lq > ws.lq_max:
+        raise ValueError("backtrack workspace too small")
+    if n == 0:
+        return DeviceAlignments(ws.out_q[:0], ws.out_t[:0], ws.info[:0], ws.A)
+    W = ws.width
+    dev = db.device
+    st = stream_of(dev)
+    call("sseqs_db_gather", ptr(db.packed), ptr(db.seq_off), ptr(db.seq_len),
+         ptr(ids.to(torch.int64).contiguous()), n, W, ptr(ws.rows), ptr(ws.lens), st)
+    cell_off = ws.t_off[:n] * Lq
+    total = Lq * n * W
+    call("sseqs_backtrack", ptr(q.qbytes), Lq, ptr(ws.rows), ptr(ws.t_off),
+         ptr(ws.lens), ptr(cell_off), total, n, int(gap_open), int(gap_extend), ws.A,
+         ptr(ws.out_q), ptr(ws.out_t), ptr(ws.info), ptr(ws.ws), ws.ws.numel(), st)
+    return DeviceAlignments(ws.out_q[:n], ws.out_t[:n], ws.info[:n], ws.A)
+
+
+def alignments_to_dicts(al: DeviceAlignments) -> list[dict]:
+    """Host dicts in the format of sseqs/sw.py:207-217."""
+    qh, th_, ih = al.out_q.cpu().numpy(), al.out_t.cpu().numpy(), al.info.cpu().numpy()
+    A = al.width
+    out = []
+    for b in range(ih.shape[0]):
+        n = int(ih[b, 0])
+        out.append({
+            "q_aligned": qh[b, A - n:].tobytes().decode("latin-1") if n else "",
+            "t_aligned": th_[b, A - n:].tobytes().decode("latin-1") if n else "",
+            "q_start_orig_0based": int(ih[b, 1]),
+            "q_end_orig_0based": int(ih[b, 2]),
+            "t_start_orig_0based": int(ih[b, 3]),
+            "t_end_orig_0based": int(ih[b, 4]),
+            "score": float(ih[b, 5]),
+        })
+    return out
+
+
 # ------------------------------------------------------------------- batch --
 
 @dataclass
