@@ -544,6 +544,41 @@ void orc_gen_db(const orc_gen_params* gp, const uint8_t* qcodes, const int64_t* 
     }
 }
 
+/* Sequences ids[0..m) only: row r of out (stride 1024) receives sequence
+ * ids[r] (ASCII, no delimiter), out_len[r] its length.  Same algorithm as
+ * orc_gen_db, for spot checks of very large databases. */
+void orc_gen_seqs(const orc_gen_params* gp, const uint8_t* qcodes, const int64_t* q_off, int nq,
+                  const int64_t* cls_start, const int32_t* cls_len, int K, const int64_t* ids,
+                  int64_t m, uint8_t* out, int32_t* out_len) {
+    #pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t r = 0; r < m; ++r) {
+        int64_t i = ids[r];
+        int cls;
+        int L = seq_length(cls_start, cls_len, K, i, &cls);
+        uint8_t* o = out + r * 1024;
+        out_len[r] = L;
+        int64_t h;
+        int hom = is_homolog(gp, i, &h) && nq > 0;
+        for (int p = 0; p < L; ++p) o[p] = gen_residue(gp, i, p);
+        if (!hom) continue;
+        int qi = (int)((uint64_t)(h * 2654435761ULL) % (uint64_t)nq);
+        const uint8_t* q = qcodes + q_off[qi];
+        int Lq = (int)(q_off[qi + 1] - q_off[qi]);
+        int mm = mutate(gp, h, q, Lq, NULL, 4096);
+        uint64_t S = (1ULL << 62) | (uint64_t)h;
+        if (mm <= L) {
+            int shift = (int)(((uint64_t)rnd24(gp->seed, S, 0) * (uint64_t)(L - mm + 1)) >> 24);
+            mutate(gp, h, q, Lq, o + shift, mm);
+        } else {
+            uint8_t* tmp = malloc((size_t)mm);
+            mutate(gp, h, q, Lq, tmp, mm);
+            int shift = (int)(((uint64_t)rnd24(gp->seed, S, 0) * (uint64_t)(mm - L + 1)) >> 24);
+            memcpy(o, tmp + shift, (size_t)L);
+            free(tmp);
+        }
+    }
+}
+
 /* Background query of length L (codes 0..19) from stream 2^60 | qid. */
 void orc_gen_query(const orc_gen_params* gp, int64_t qid, int L, uint8_t* codes) {
     for (int p = 0; p < L; ++p)

@@ -400,3 +400,23 @@ def rank_hits(hits, num_msas):
     """sseqs/msa.py:426-427 with ties in collection order."""
     s = np.array([h[0] for h in hits], dtype=np.float64)
     return [hits[i] for i in np.argsort(-s, kind="stable")[:num_msas]]
+
+
+def gen_seqs(params, queries: list[str], cls_start, cls_len, ids) -> list[str]:
+    """Selected sequences of the synthetic database (same algorithm as
+    gen_db), for spot checks of databases too large to restate whole."""
+    codes = [np.array([AA_TO_POS[c] for c in q], np.uint8) for q in queries]
+    q_off = np.concatenate([[0], np.cumsum([c.size for c in codes])]).astype(np.int64)
+    qc = np.concatenate(codes) if codes else np.zeros(1, np.uint8)
+    cs = np.ascontiguousarray(cls_start, np.int64)
+    cl = np.ascontiguousarray(cls_len, np.int32)
+    ids = np.ascontiguousarray(ids, np.int64)
+    out = np.zeros((ids.size, 1024), np.uint8)
+    ln = np.zeros(ids.size, np.int32)
+    L = lib()
+    L.orc_gen_seqs.argtypes = [C.c_void_p] * 4 + [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                                  C.c_int64, C.c_void_p, C.c_void_p]
+    L.orc_gen_seqs.argtypes[3] = C.c_int
+    L.orc_gen_seqs(C.addressof(params), _p(qc), _p(q_off), len(queries), _p(cs), _p(cl), cl.size,
+                   _p(ids), ids.size, _p(out), _p(ln))
+    return [out[r, : ln[r]].tobytes().decode("latin-1") for r in range(ids.size)]

@@ -65,4 +65,13 @@ __device__ inline int wave_max_i32(int v) {
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Blocks for a grid-stride launch over `items` units of `per_block` each.
+// An HSA dispatch counts work-items in 32 bits, so kernels that may cover a
+// whole database loop instead of launching one thread per unit; 65536
+// blocks fill the 256 CUs many times over.
+inline unsigned grid_cap(int64_t items, int64_t per_block) {
+    int64_t b = ceil_div(items > 0 ? items : 1, per_block);
+    return (unsigned)(b < 65536 ? b : 65536);
+}
+
 }  // namespace sseqs

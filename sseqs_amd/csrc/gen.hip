@@ -113,12 +113,10 @@ __device__ int mutate(const GenCtx& c, int64_t h, const uint8_t* q, int Lq, uint
     return n;
 }
 
-// One thread per homolog: background residues, then the mutated query copy
+// Homolog of block h: background residues, then the mutated query copy
 // (whole copy at a random shift when it fits, else a random window of it).
-__global__ __launch_bounds__(256) void k_gen_homologs(GenCtx c, uint8_t* __restrict__ hs) {
-    int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ void gen_homolog(const GenCtx& c, uint8_t* __restrict__ hs, int64_t h) {
     const int P = c.gp.hom_period;
-    if (h * P >= c.n || c.nq <= 0) return;
     int64_t slot = (int64_t)(mix64(c.gp.seed ^ (0xA5A5A5A5ULL + (uint64_t)h)) % (uint64_t)P);
     int64_t i = h * P + slot;
     if (i >= c.n) return;
@@ -141,12 +139,20 @@ __global__ __launch_bounds__(256) void k_gen_homologs(GenCtx c, uint8_t* __restr
     }
 }
 
-// One thread per 8-symbol group of the packed output.
-__global__ __launch_bounds__(256) void k_gen_pack(GenCtx c, const uint8_t* __restrict__ hs,
-                                                  uint8_t* __restrict__ packed) {
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Grid-stride over homolog blocks (an HSA dispatch counts work-items in 32
+// bits, so database-sized launches loop instead of one thread per unit).
+__global__ __launch_bounds__(256) void k_gen_homologs(GenCtx c, uint8_t* __restrict__ hs) {
+    if (c.nq <= 0) return;
+    const int P = c.gp.hom_period;
+    for (int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; h * P < c.n;
+         h += (int64_t)gridDim.x * blockDim.x)
+        gen_homolog(c, hs, h);
+}
+
+// One 8-symbol group of the packed output.
+__device__ void gen_group(const GenCtx& c, const uint8_t* __restrict__ hs,
+                          uint8_t* __restrict__ packed, int64_t g) {
     int64_t q0 = g * 8;
-    if (q0 >= c.n_syms) return;
     int cls = class_of_sym(c, q0);
     int L = c.cls_len[cls];
     int64_t rel = q0 - c.cls_sym[cls];
@@ -184,11 +190,16 @@ __global__ __launch_bounds__(256) void k_gen_pack(GenCtx c, const uint8_t* __res
     }
 }
 
-__global__ __launch_bounds__(256) void k_gen_index(GenCtx c, int64_t* __restrict__ off,
-                                                   int32_t* __restrict__ len,
-                                                   int32_t* __restrict__ homq) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= c.n) return;
+__global__ __launch_bounds__(256) void k_gen_pack(GenCtx c, const uint8_t* __restrict__ hs,
+                                                  uint8_t* __restrict__ packed) {
+    const int64_t groups = (c.n_syms + 7) / 8;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < groups;
+         g += (int64_t)gridDim.x * blockDim.x)
+        gen_group(c, hs, packed, g);
+}
+
+__device__ void gen_index_one(const GenCtx& c, int64_t i, int64_t* __restrict__ off,
+                              int32_t* __restrict__ len, int32_t* __restrict__ homq) {
     int cls = class_of_seq(c, i);
     int L = c.cls_len[cls];
     off[i] = c.cls_sym[cls] + (i - c.cls_start[cls]) * (int64_t)(L + 1);
@@ -196,6 +207,14 @@ __global__ __launch_bounds__(256) void k_gen_index(GenCtx c, int64_t* __restrict
     int64_t blk;
     bool hom = homolog_of(c, i, blk);
     if (homq) homq[i] = hom ? (int32_t)((uint64_t)(blk * 2654435761ULL) % (uint64_t)c.nq) : -1;
+}
+
+__global__ __launch_bounds__(256) void k_gen_index(GenCtx c, int64_t* __restrict__ off,
+                                                   int32_t* __restrict__ len,
+                                                   int32_t* __restrict__ homq) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < c.n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        gen_index_one(c, i, off, len, homq);
 }
 
 }  // namespace sseqs
@@ -231,14 +250,13 @@ extern "C" int sseqs_gen_db(const sseqs_gen_params* h_params, const uint8_t* qco
     uint8_t* hs = static_cast<uint8_t*>(ws);
     int64_t nh = ceil_div(n, h_params->hom_period);
     if (nq > 0) {
-        hipLaunchKernelGGL(k_gen_homologs, dim3((unsigned)ceil_div(nh, 256)), dim3(256), 0, stream, c,
-                           hs);
+        hipLaunchKernelGGL(k_gen_homologs, dim3(grid_cap(nh, 256)), dim3(256), 0, stream, c, hs);
         if (int r = check_launch("gen homologs")) return r;
     }
-    hipLaunchKernelGGL(k_gen_pack, dim3((unsigned)ceil_div(ceil_div(n_syms, 8), 256)), dim3(256), 0,
-                       stream, c, hs, packed);
+    hipLaunchKernelGGL(k_gen_pack, dim3(grid_cap(ceil_div(n_syms, 8), 256)), dim3(256), 0, stream,
+                       c, hs, packed);
     if (int r = check_launch("gen pack")) return r;
-    hipLaunchKernelGGL(k_gen_index, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, c,
+    hipLaunchKernelGGL(k_gen_index, dim3(grid_cap(n, 256)), dim3(256), 0, stream, c,
                        seq_off, seq_len, hom_query);
     return check_launch("gen index");
 }

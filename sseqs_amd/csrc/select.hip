@@ -82,10 +82,12 @@ __global__ __launch_bounds__(1024) void k_sel_pick(SelState* st, uint32_t* hist,
 }
 
 __global__ __launch_bounds__(256) void k_sel_count(const void* s, int dtype, int64_t n,
-                                                   const SelState* st, int64_t* cgt,
+                                                   int64_t nb, const SelState* st, int64_t* cgt,
                                                    int64_t* ceq) {
     const uint32_t T = st->prefix;
-    int64_t base = (int64_t)blockIdx.x * SEL_TILE;
+    __shared__ int rg[256], re[256];
+    for (int64_t tile = blockIdx.x; tile < nb; tile += gridDim.x) {
+    const int64_t base = tile * SEL_TILE;
     int gt = 0, eq = 0;
     for (int j = threadIdx.x; j < SEL_TILE; j += 256) {
         int64_t i = base + j;
@@ -95,7 +97,6 @@ __global__ __launch_bounds__(256) void k_sel_count(const void* s, int dtype, int
             eq += key == T;
         }
     }
-    __shared__ int rg[256], re[256];
     rg[threadIdx.x] = gt;
     re[threadIdx.x] = eq;
     __syncthreads();
@@ -107,8 +108,10 @@ __global__ __launch_bounds__(256) void k_sel_count(const void* s, int dtype, int
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        cgt[blockIdx.x] = rg[0];
-        ceq[blockIdx.x] = re[0];
+        cgt[tile] = rg[0];
+        ceq[tile] = re[0];
+    }
+    __syncthreads();
     }
 }
 
@@ -144,14 +147,16 @@ __global__ __launch_bounds__(1024) void k_sel_scan(int64_t* a, int64_t* b, int64
 }
 
 __global__ __launch_bounds__(256) void k_sel_write(const void* s, int dtype, int64_t n,
-                                                   const SelState* st, const int64_t* bgt,
-                                                   const int64_t* beq, int64_t k, int64_t* out) {
+                                                   int64_t nb, const SelState* st,
+                                                   const int64_t* bgt, const int64_t* beq,
+                                                   int64_t k, int64_t* out) {
     const uint32_t T = st->prefix;
     const int64_t keq = st->k_rem;
-    int64_t gt_run = bgt[blockIdx.x], eq_run = beq[blockIdx.x];
-    const int64_t base = (int64_t)blockIdx.x * SEL_TILE;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     __shared__ int wg[4], we[4];
+    for (int64_t tile = blockIdx.x; tile < nb; tile += gridDim.x) {
+    int64_t gt_run = bgt[tile], eq_run = beq[tile];
+    const int64_t base = tile * SEL_TILE;
     for (int j0 = 0; j0 < SEL_TILE; j0 += 256) {
         int64_t i = base + j0 + threadIdx.x;
         uint32_t key = i < n ? load_key(s, dtype, i) : 0u;
@@ -179,6 +184,7 @@ __global__ __launch_bounds__(256) void k_sel_write(const void* s, int dtype, int
         gt_run += tg;
         eq_run += te;
         __syncthreads();
+    }
     }
 }
 
@@ -254,14 +260,14 @@ extern "C" int sseqs_topk(const void* scores, int dtype, int64_t n, int64_t k, i
         hipLaunchKernelGGL(k_sel_pick, dim3(1), dim3(1024), 0, stream, st, hist, shifts[p], bits[p]);
         if (int r = check_launch("topk pick")) return r;
     }
-    hipLaunchKernelGGL(k_sel_count, dim3((unsigned)nb), dim3(256), 0, stream, scores, dtype, n, st,
-                       cgt, ceq);
+    hipLaunchKernelGGL(k_sel_count, dim3(grid_cap(nb, 1)), dim3(256), 0, stream, scores, dtype, n,
+                       nb, st, cgt, ceq);
     if (int r = check_launch("topk count")) return r;
     hipLaunchKernelGGL(k_sel_scan, dim3(1), dim3(1024), 0, stream, cgt, ceq, nb);
     if (int r = check_launch("topk scan")) return r;
     int64_t* dst = sorted ? tmp : out_idx;
-    hipLaunchKernelGGL(k_sel_write, dim3((unsigned)nb), dim3(256), 0, stream, scores, dtype, n, st,
-                       cgt, ceq, k, dst);
+    hipLaunchKernelGGL(k_sel_write, dim3(grid_cap(nb, 1)), dim3(256), 0, stream, scores, dtype, n,
+                       nb, st, cgt, ceq, k, dst);
     if (int r = check_launch("topk write")) return r;
     if (sorted) {
         int P = 1;

@@ -7,44 +7,50 @@
 
 namespace sseqs {
 
-// Each thread unpacks one 8-symbol group: 5 byte loads (the group is not
-// aligned), one 8-byte store when the group is whole.
+// One 8-symbol group per thread and step: 5 byte loads (the group is not
+// aligned), one 8-byte store when the group is whole.  Grid-stride: an HSA
+// dispatch counts work-items in 32 bits, so a database-sized stream
+// (> 2^32 groups) must not be one thread per group.
 __global__ __launch_bounds__(256) void k_unpack(const uint8_t* __restrict__ in,
                                                 uint8_t* __restrict__ out, int64_t n) {
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t q0 = g * 8;
-    if (q0 >= n) return;
-    const uint8_t* p = in + g * 5;
-    uint64_t w = ((uint64_t)p[0] << 32) | ((uint64_t)p[1] << 24) | ((uint64_t)p[2] << 16) |
-                 ((uint64_t)p[3] << 8) | (uint64_t)p[4];
-    uint64_t o = 0;
+    const int64_t groups = (n + 7) / 8;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < groups;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q0 = g * 8;
+        const uint8_t* p = in + g * 5;
+        uint64_t w = ((uint64_t)p[0] << 32) | ((uint64_t)p[1] << 24) | ((uint64_t)p[2] << 16) |
+                     ((uint64_t)p[3] << 8) | (uint64_t)p[4];
+        uint64_t o = 0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) o |= (((w >> (35 - 5 * k)) & 31u) + 64u) << (8 * k);
-    if (q0 + 8 <= n && ((reinterpret_cast<uintptr_t>(out + q0) & 7) == 0)) {
-        *reinterpret_cast<uint64_t*>(out + q0) = o;
-    } else {
-        for (int k = 0; k < 8 && q0 + k < n; ++k) out[q0 + k] = (uint8_t)(o >> (8 * k));
+        for (int k = 0; k < 8; ++k) o |= (((w >> (35 - 5 * k)) & 31u) + 64u) << (8 * k);
+        if (q0 + 8 <= n && ((reinterpret_cast<uintptr_t>(out + q0) & 7) == 0)) {
+            *reinterpret_cast<uint64_t*>(out + q0) = o;
+        } else {
+            for (int k = 0; k < 8 && q0 + k < n; ++k) out[q0 + k] = (uint8_t)(o >> (8 * k));
+        }
     }
 }
 
-// Each thread packs one group; a short final group is padded with '@' (0)
-// and only bytes below ceil(5n/8) are written.
+// One group per thread and step; a short final group is padded with '@'
+// (0) and only bytes below ceil(5n/8) are written.
 __global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ in,
                                               uint8_t* __restrict__ out, int64_t n) {
-    int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t q0 = g * 8;
-    if (q0 >= n) return;
-    uint64_t w = 0;
+    const int64_t groups = (n + 7) / 8;
+    const int64_t nout = (n * 5 + 7) / 8;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < groups;
+         g += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q0 = g * 8;
+        uint64_t w = 0;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        uint64_t s = (q0 + k < n) ? (uint64_t)((uint8_t)(in[q0 + k] - 64) & 31u) : 0u;
-        w |= s << (35 - 5 * k);
-    }
-    int64_t nout = (n * 5 + 7) / 8;
+        for (int k = 0; k < 8; ++k) {
+            uint64_t s = (q0 + k < n) ? (uint64_t)((uint8_t)(in[q0 + k] - 64) & 31u) : 0u;
+            w |= s << (35 - 5 * k);
+        }
 #pragma unroll
-    for (int b = 0; b < 5; ++b) {
-        int64_t o = g * 5 + b;
-        if (o < nout) out[o] = (uint8_t)(w >> (32 - 8 * b));
+        for (int b = 0; b < 5; ++b) {
+            int64_t o = g * 5 + b;
+            if (o < nout) out[o] = (uint8_t)(w >> (32 - 8 * b));
+        }
     }
 }
 
@@ -62,21 +68,24 @@ __device__ inline bool run_start(const uint8_t* p, int64_t q) {
 }
 
 __global__ __launch_bounds__(256) void k_idx_count(const uint8_t* __restrict__ p, int64_t n,
-                                                   int64_t* __restrict__ tile_cnt) {
-    int64_t base = (int64_t)blockIdx.x * IDX_TILE;
-    int c = 0;
-    for (int i = threadIdx.x; i < IDX_TILE; i += 256) {
-        int64_t q = base + i;
-        if (q < n && run_start(p, q)) ++c;
-    }
+                                                   int64_t nt, int64_t* __restrict__ tile_cnt) {
     __shared__ int red[256];
-    red[threadIdx.x] = c;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+    for (int64_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+        const int64_t base = tile * IDX_TILE;
+        int c = 0;
+        for (int i = threadIdx.x; i < IDX_TILE; i += 256) {
+            int64_t q = base + i;
+            if (q < n && run_start(p, q)) ++c;
+        }
+        red[threadIdx.x] = c;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) tile_cnt[tile] = red[0];
         __syncthreads();
     }
-    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = red[0];
 }
 
 // Exclusive scan of ntile counts in one block of 1024 threads; total -> *total.
@@ -105,13 +114,14 @@ __global__ __launch_bounds__(1024) void k_scan_i64(int64_t* __restrict__ v, int6
 }
 
 __global__ __launch_bounds__(256) void k_idx_write(const uint8_t* __restrict__ p, int64_t n,
-                                                   const int64_t* __restrict__ tile_base,
+                                                   int64_t nt, const int64_t* __restrict__ tile_base,
                                                    int64_t* __restrict__ off,
                                                    int32_t* __restrict__ len, int64_t cap) {
-    int64_t base = (int64_t)blockIdx.x * IDX_TILE;
     __shared__ int wsum[4];
-    int64_t out = tile_base[blockIdx.x];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    for (int64_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+    const int64_t base = tile * IDX_TILE;
+    int64_t out = tile_base[tile];
     for (int i0 = 0; i0 < IDX_TILE; i0 += 256) {
         int64_t q = base + i0 + threadIdx.x;
         bool st = q < n && run_start(p, q);
@@ -133,6 +143,7 @@ __global__ __launch_bounds__(256) void k_idx_write(const uint8_t* __restrict__ p
         }
         out += tot;
         __syncthreads();
+    }
     }
 }
 
@@ -173,8 +184,7 @@ extern "C" int decompress_launcher(const uint8_t* d_in, uint8_t* d_out, int64_t 
     if (n_syms < 0 || lo < 0) return fail(-1, "decompress: negative size or offset");
     if (n_syms == 0) return 0;
     if (!d_in || !d_out) return fail(-1, "decompress: null pointer");
-    int64_t groups = ceil_div(n_syms, 8);
-    hipLaunchKernelGGL(k_unpack, dim3((unsigned)ceil_div(groups, 256)), dim3(256), 0, stream,
+    hipLaunchKernelGGL(k_unpack, dim3(grid_cap(ceil_div(n_syms, 8), 256)), dim3(256), 0, stream,
                        d_in + lo, d_out, n_syms);
     return check_launch("decompress");
 }
@@ -184,8 +194,7 @@ extern "C" int compress_launcher(const uint8_t* d_in, uint8_t* d_out, int64_t n_
     if (n_syms < 0) return fail(-1, "compress: negative size");
     if (n_syms == 0) return 0;
     if (!d_in || !d_out) return fail(-1, "compress: null pointer");
-    int64_t groups = ceil_div(n_syms, 8);
-    hipLaunchKernelGGL(k_pack, dim3((unsigned)ceil_div(groups, 256)), dim3(256), 0, stream, d_in,
+    hipLaunchKernelGGL(k_pack, dim3(grid_cap(ceil_div(n_syms, 8), 256)), dim3(256), 0, stream, d_in,
                        d_out, n_syms);
     return check_launch("compress");
 }
@@ -200,13 +209,13 @@ extern "C" int sseqs_db_index(const uint8_t* packed, int64_t n_syms, int64_t* se
     if (n_syms <= 0) return fail(-1, "db_index: empty database");
     if (ws_bytes < sseqs_db_index_ws_bytes(n_syms)) return fail(-2, "db_index: workspace too small");
     int64_t nt = ceil_div(n_syms, IDX_TILE);
-    if (nt > 0x7fffffffLL) return fail(-1, "db_index: database too large");
     int64_t* tiles = static_cast<int64_t*>(ws);
-    hipLaunchKernelGGL(k_idx_count, dim3((unsigned)nt), dim3(256), 0, stream, packed, n_syms, tiles);
+    hipLaunchKernelGGL(k_idx_count, dim3(grid_cap(nt, 1)), dim3(256), 0, stream, packed, n_syms, nt,
+                       tiles);
     if (int r = check_launch("db_index count")) return r;
     hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, stream, tiles, nt, d_nseq);
     if (int r = check_launch("db_index scan")) return r;
-    hipLaunchKernelGGL(k_idx_write, dim3((unsigned)nt), dim3(256), 0, stream, packed, n_syms, tiles,
-                       seq_off, seq_len, cap);
+    hipLaunchKernelGGL(k_idx_write, dim3(grid_cap(nt, 1)), dim3(256), 0, stream, packed, n_syms, nt,
+                       tiles, seq_off, seq_len, cap);
     return check_launch("db_index write");
 }

@@ -1,0 +1,85 @@
+"""Size-independent checks at the benchmark's full size (200 M sequences,
+56.6 G residues, 35.5 GB packed): every launch that covers the database
+must reach its end (an HSA dispatch counts work-items in 32 bits, so one
+thread per 8-symbol group would stop at 2^32 groups = 34.4 G symbols).
+
+* the delimiter scan of the generated stream finds exactly n sequences and
+  reproduces the generator's index;
+* sequences sampled across the whole database (including past 2^32 groups
+  and the very last ones) equal the CPU generator's;
+* the whole-database cascade for the 500-residue query: prefilter scores of
+  a random sample within tolerance of the FP64 closed form, SW scores of
+  its hits bit-exact, and the hits are the query's planted homologs.
+"""
+import numpy as np
+import pytest
+import torch
+
+from conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+N_FULL = 200_000_000
+
+
+@pytest.fixture(scope="module")
+def big(dev):
+    from sseqs_amd import synth
+    from sseqs_amd.db import SeqDB
+    announce("generate 200M-sequence database")
+    q200 = synth.make_query(200, 0, seed=1)
+    q500 = synth.make_query(500, 1, seed=1)
+    db = SeqDB.synthetic(N_FULL, [q200, q500], seed=0, device=dev)
+    torch.cuda.synchronize()
+    yield dict(db=db, queries=[q200, q500])
+    del db
+    torch.cuda.empty_cache()
+
+
+def test_full_index_and_content(orc, big):
+    from sseqs_amd import synth
+    from sseqs_amd.db import SeqDB
+    db = big["db"]
+    announce("delimiter scan of the full stream")
+    scan = SeqDB.from_packed(db.packed, db.n_syms)
+    assert scan.n == db.n and scan.orig_id is None
+    assert torch.equal(scan.seq_off, db.seq_off) and torch.equal(scan.seq_len, db.seq_len)
+    del scan
+    rng = np.random.default_rng(1)
+    off = db.seq_off.cpu().numpy()
+    past = np.nonzero(off >= (1 << 32) * 8)[0]
+    ids = np.unique(np.concatenate([rng.choice(db.n, 1500, replace=False),
+                                    np.arange(db.n - 50, db.n), past[:20],
+                                    rng.choice(past, 300, replace=False)]))
+    cls = db.extra["classes"]
+    want = orc.gen_seqs(synth.gen_params(0), big["queries"], cls.start, cls.length, ids)
+    got = db.strings(torch.from_numpy(ids).to(db.device), 1024)
+    assert got == want
+
+
+def test_full_cascade_q500(orc, big):
+    from sseqs_amd.query import prepare
+    from sseqs_amd.search import global_search
+    db, query = big["db"], big["queries"][1]
+    announce("global cascade q_len=500 over 200M sequences")
+    q = prepare(query)
+    gh = global_search(db, q, top_prefilter=100_000, top_hits=4096)
+    ids = gh.ids.cpu().numpy()
+    hq = db.extra["hom_query"].cpu().numpy()[ids]
+    assert np.mean(hq[:1000] == 1) > 0.95
+    # SW of the hits against the oracle
+    sel = ids[:64]
+    rows = db.strings(torch.from_numpy(sel).to(db.device), 1024)
+    qc = orc.encode_query_sw(query)
+    assert gh.scores[:64].cpu().tolist() == [orc.sw_affine_one(qc, r) for r in rows]
+    # prefilter of a random sample against FP64
+    rng = np.random.default_rng(2)
+    samp = np.sort(rng.choice(db.n, 400, replace=False))
+    rows = db.strings(torch.from_numpy(samp).to(db.device), 1024)
+    A, B = orc.weighted_luts(query, 4)
+    qrev = orc.encode_query_rev(query)
+    want = np.array([orc.corr(qrev, A, B, 4, r.encode(), [0], [len(r)], 10**6,
+                              orc.nextpow2(max(len(query), len(r), 16)))[0] for r in rows])
+    got = gh.prefilter.cpu().numpy()[samp]
+    scale = 1e-3 * np.max(np.abs(want))
+    assert np.all(np.abs(got - want) <= 1e-2 * np.abs(want) + scale)
