@@ -183,6 +183,146 @@ __global__ __launch_bounds__(64) void k_corr(Src src, int64_t s_begin, int64_t n
     }
 }
 
+// ---------------------------------------------------------------------------
+// Database form: fused unpack + prefilter over the packed, HBM-resident
+// stream.  A tile's 32 sequences are adjacent in a length-sorted database,
+// so their packed bytes form one short range that the wave copies into LDS
+// with coalesced 16-byte loads; B fragments are then decoded straight from
+// that copy (8 symbols = one 40-bit window, read as two aligned 8-byte LDS
+// words and byte-swapped to big-endian), once per 16-column chunk and
+// shared by all ranks.  A tile whose range does not fit (an index that is
+// not in stream order) reads its symbols from global memory instead.
+// Dynamic LDS: qtab k*(N+8) fp16 | btab k*40 fp16 | stage CAPB+64 bytes.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_corr_db(const uint8_t* __restrict__ packed,
+                                                const int64_t* __restrict__ off,
+                                                const int32_t* __restrict__ len, int64_t s_begin,
+                                                int64_t nseq, const uint8_t* __restrict__ qrev,
+                                                int Lq, const float* __restrict__ lutA,
+                                                const float* __restrict__ lutB, int V, int k,
+                                                int N, int maxlen, int CAPB,
+                                                float* __restrict__ out) {
+    extern __shared__ __align__(16) uint8_t smem[];
+    _Float16* qtab = reinterpret_cast<_Float16*>(smem);
+    const int QS = N + 8;
+    _Float16* btab = qtab + k * QS;
+    uint8_t* stg = reinterpret_cast<uint8_t*>(btab + k * CORR_NB);
+
+    const int lane = threadIdx.x;
+    const int Lqe = min(Lq, N);
+    for (int idx = lane; idx < k * QS; idx += 64) {
+        int r = idx / QS, u = idx - r * QS;
+        int t = (N - (u & (N - 1))) & (N - 1);
+        qtab[idx] = (_Float16)(t < Lqe ? 0.5f * lutA[r * V + (int)qrev[t]] : 0.f);
+    }
+    for (int idx = lane; idx < k * CORR_NB; idx += 64) {
+        int r = idx / CORR_NB, c = idx - r * CORR_NB;
+        btab[idx] = (_Float16)(c < 32 ? lutB[r * V + corr_code((uint32_t)c + 64u)] : 0.f);
+    }
+
+    const int sl = lane & 31, h = lane >> 5;
+    const int NMT = N >= 64 ? N / 64 : 1;
+    for (int64_t tile = blockIdx.x; tile * 32 < nseq; tile += gridDim.x) {
+        const int64_t sid = tile * 32 + sl;
+        const bool in = sid < nseq;
+        int64_t o = 0;
+        int Le = 0;
+        if (in) {
+            o = off[s_begin + sid];
+            Le = max(0, min(min(len[s_begin + sid], maxlen), N));
+        }
+        int kp = Le;
+        int64_t lo = in ? o : INT64_MAX, hi = in ? o : 0;
+#pragma unroll
+        for (int d = 16; d > 0; d >>= 1) {
+            kp = max(kp, __shfl_xor(kp, d, 64));
+            lo = min(lo, __shfl_xor(lo, d, 64));
+            hi = max(hi, __shfl_xor(hi, d, 64));
+        }
+        kp = (kp + 15) & ~15;
+        if (!in) o = lo;  // idle lanes decode (and mask) inside the stage
+        // Symbols read: [lo, hi + kp).  The last window of a lane starts at
+        // byte 5*(o+kp-8)/8 and spans 16 bytes, so 24 bytes past 5*(hi+kp)/8
+        // cover every read; the start is 16-byte aligned for the copy.
+        const int64_t b_lo = ((lo * 5) >> 3) & ~(int64_t)15;
+        const int64_t b_hi = (((hi + kp) * 5) >> 3) + 24;
+        const bool fast = (b_hi - b_lo) <= CAPB;
+        __syncthreads();  // the previous tile no longer reads the stage
+        if (fast) {
+            const uint4* src = reinterpret_cast<const uint4*>(packed + b_lo);
+            uint4* dst = reinterpret_cast<uint4*>(stg);
+            const int nchunk = (int)((b_hi - b_lo + 15) >> 4);
+            for (int c = lane; c < nchunk; c += 64) dst[c] = src[c];
+        }
+        __syncthreads();
+        const int64_t relbit = 5 * o - 8 * b_lo;  // bit of this sequence in the stage
+
+        float best = -INFINITY;
+        for (int mp = 0; mp < NMT; mp += CORR_MT) {
+            f32x16 acc[CORR_MT];
+#pragma unroll
+            for (int t = 0; t < CORR_MT; ++t)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) acc[t][e] = 0.f;
+            for (int p0 = 0; p0 < kp; p0 += 16) {
+                const int pb = p0 + 8 * h;
+                uint32_t sym[8];
+                if (fast) {
+                    const int64_t bit = relbit + 5 * (int64_t)pb;
+                    const int64_t byte = bit >> 3;
+                    const uint64_t* s64 = reinterpret_cast<const uint64_t*>(stg + (byte & ~7));
+                    const uint64_t B0 = __builtin_bswap64(s64[0]);
+                    const uint64_t B1 = __builtin_bswap64(s64[1]);
+                    const int sh = (int)(byte & 7) * 8 + (int)(bit & 7);
+                    const uint64_t top = sh ? (B0 << sh) | (B1 >> (64 - sh)) : B0;
+#pragma unroll
+                    for (int x = 0; x < 8; ++x) sym[x] = (uint32_t)(top >> (59 - 5 * x)) & 31u;
+                } else {
+#pragma unroll
+                    for (int x = 0; x < 8; ++x)
+                        sym[x] = pb + x < Le ? packed_sym(packed, o + pb + x) - 64u : 0u;
+                }
+#pragma unroll
+                for (int x = 0; x < 8; ++x) sym[x] = pb + x < Le ? sym[x] : (uint32_t)CORR_PAD;
+                for (int r = 0; r < k; ++r) {
+                    const _Float16* br = btab + r * CORR_NB;
+                    const _Float16* qr = qtab + r * QS;
+                    half8 bfrag;
+#pragma unroll
+                    for (int x = 0; x < 8; ++x) bfrag[x] = br[sym[x]];
+#pragma unroll
+                    for (int t = 0; t < CORR_MT; ++t) {
+                        if (mp + t < NMT) {
+                            const int i = (mp + t) * 32 + sl;
+                            const int u0 = (pb - 2 * i) & (N - 1);
+                            const uint32_t* q32 = reinterpret_cast<const uint32_t*>(qr + u0);
+                            union {
+                                uint32_t w[4];
+                                half8 v;
+                            } a;
+                            a.w[0] = q32[0];
+                            a.w[1] = q32[1];
+                            a.w[2] = q32[2];
+                            a.w[3] = q32[3];
+                            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a.v, bfrag, acc[t], 0,
+                                                                            0, 0);
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < CORR_MT; ++t) {
+                if (mp + t < NMT) {
+#pragma unroll
+                    for (int e = 0; e < 16; ++e) best = fmaxf(best, acc[t][e]);
+                }
+            }
+        }
+        best = fmaxf(best, __shfl_xor(best, 32, 64));
+        if (h == 0 && in) out[sid] = best;
+    }
+}
+
 template <class Src, class OutT, bool PACKED_CODES>
 static int launch_corr(const Src& src, int64_t s_begin, int64_t nseq, const uint8_t* qrev, int Lq,
                        const float* lutA, const float* lutB, int V, int k, int N, int maxlen,
@@ -233,8 +373,28 @@ extern "C" int sseqs_db_prefilter(const uint8_t* packed, const int64_t* seq_off,
                                   const float* lutB, int vocab_size, int k, int N, int maxlen,
                                   float* scores, hipStream_t stream) {
     if (s_end < s_begin) return fail(-1, "db_prefilter: s_end < s_begin");
-    CorrPacked src{packed, seq_off, seq_len};
-    return launch_corr<CorrPacked, float, true>(src, s_begin, s_end - s_begin, qrev, query_len,
-                                                lutA, lutB, vocab_size, k, N, maxlen, scores,
-                                                stream);
+    const int64_t nseq = s_end - s_begin;
+    if (nseq == 0) return 0;
+    if (N < 16 || N > 2048 || (N & (N - 1)))
+        return fail(-1, "db_prefilter: N must be a power of two in 16..2048");
+    if (k < 1 || k > 20) return fail(-1, "db_prefilter: k must be 1..20");
+    if (vocab_size < 1) return fail(-1, "db_prefilter: vocab_size must be >= 1");
+    const int mlen = maxlen < 0 ? 0 : maxlen;
+    const int KPmax = (min(mlen, N) + 15) & ~15;
+    // a length-sorted tile spans about 32 sequences of at most KPmax symbols
+    int CAPB = (int)(((int64_t)33 * (KPmax + 17) * 5) / 8 + 64);
+    CAPB = (CAPB + 15) & ~15;
+    size_t lds = (size_t)k * (N + 8) * 2 + (size_t)k * CORR_NB * 2 + (size_t)CAPB + 64;
+    lds = (lds + 15) & ~(size_t)15;
+    if (lds > 160 * 1024) return fail(-1, "db_prefilter: tables exceed LDS (reduce k or N)");
+    if (lds > 64 * 1024 &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_corr_db),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+        return fail(-3, "db_prefilter: cannot enable large LDS");
+    const int64_t tiles = ceil_div(nseq, 32);
+    const unsigned grid = (unsigned)(tiles < 8192 ? tiles : 8192);
+    hipLaunchKernelGGL(k_corr_db, dim3(grid), dim3(64), lds, stream, packed, seq_off, seq_len,
+                       s_begin, nseq, qrev, query_len, lutA, lutB, vocab_size, k, N, mlen, CAPB,
+                       scores);
+    return check_launch("db_prefilter");
 }

@@ -21,7 +21,9 @@ import torch
 from . import _lib
 from ._lib import call, ptr, stream_of
 
-PAD = 16  # tail bytes so that 2-byte symbol reads never leave the buffer
+# Tail bytes after the stream: symbol reads touch 2 bytes and the prefilter
+# copies 16-byte chunks up to 40 bytes past the last symbol's byte.
+PAD = 64
 
 
 @dataclass

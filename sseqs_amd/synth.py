@@ -159,7 +159,7 @@ def encode_codes(seq: str) -> np.ndarray:
 
 @dataclass
 class SyntheticDB:
-    packed: torch.Tensor     # uint8, ceil(5*n_syms/8) + 16 padding bytes
+    packed: torch.Tensor     # uint8, ceil(5*n_syms/8) + db.PAD padding bytes
     seq_off: torch.Tensor    # int64 [n]
     seq_len: torch.Tensor    # int32 [n]
     hom_query: torch.Tensor  # int32 [n], query id or -1
@@ -182,7 +182,8 @@ def generate_db(n: int, queries: list[str], seed: int = 0, device="cuda",
     d_q, d_qoff = t(qc), t(q_off)
     d_start, d_len, d_sym = t(cls.start), t(cls.length), t(cls.sym[:-1].copy())
     n_syms = cls.n_syms
-    packed = torch.zeros((n_syms * 5 + 7) // 8 + 16, dtype=torch.uint8, device=dev)
+    from .db import PAD
+    packed = torch.zeros((n_syms * 5 + 7) // 8 + PAD, dtype=torch.uint8, device=dev)
     seq_off = torch.empty(n, dtype=torch.int64, device=dev)
     seq_len = torch.empty(n, dtype=torch.int32, device=dev)
     homq = torch.empty(n, dtype=torch.int32, device=dev)
