@@ -194,6 +194,7 @@ __global__ __launch_bounds__(64) void k_corr(Src src, int64_t s_begin, int64_t n
 // not in stream order) reads its symbols from global memory instead.
 // Dynamic LDS: qtab k*(N+8) fp16 | btab k*40 fp16 | stage CAPB+64 bytes.
 // ---------------------------------------------------------------------------
+template <int MT>  // m-tiles (32 even shifts each) accumulated per pass; NMT % MT == 0
 __global__ __launch_bounds__(64) void k_corr_db(const uint8_t* __restrict__ packed,
                                                 const int64_t* __restrict__ off,
                                                 const int32_t* __restrict__ len, int64_t s_begin,
@@ -258,10 +259,10 @@ __global__ __launch_bounds__(64) void k_corr_db(const uint8_t* __restrict__ pack
         const int64_t relbit = 5 * o - 8 * b_lo;  // bit of this sequence in the stage
 
         float best = -INFINITY;
-        for (int mp = 0; mp < NMT; mp += CORR_MT) {
-            f32x16 acc[CORR_MT];
+        for (int mp = 0; mp < NMT; mp += MT) {
+            f32x16 acc[MT];
 #pragma unroll
-            for (int t = 0; t < CORR_MT; ++t)
+            for (int t = 0; t < MT; ++t)
 #pragma unroll
                 for (int e = 0; e < 16; ++e) acc[t][e] = 0.f;
             for (int p0 = 0; p0 < kp; p0 += 16) {
@@ -284,39 +285,241 @@ __global__ __launch_bounds__(64) void k_corr_db(const uint8_t* __restrict__ pack
                 }
 #pragma unroll
                 for (int x = 0; x < 8; ++x) sym[x] = pb + x < Le ? sym[x] : (uint32_t)CORR_PAD;
-                for (int r = 0; r < k; ++r) {
+                // A-fragment base for m-tile 0 of this pass; m-tile t is 64*t earlier
+                const int ubase = pb - 2 * (mp * 32 + sl);
+                // operands of rank r are loaded while the MFMAs of rank r-1 run
+                half8 bcur, bnxt;
+                half8 acur[MT], anxt[MT];
+                auto load = [&](int r, half8& b, half8* a) {
                     const _Float16* br = btab + r * CORR_NB;
                     const _Float16* qr = qtab + r * QS;
-                    half8 bfrag;
 #pragma unroll
-                    for (int x = 0; x < 8; ++x) bfrag[x] = br[sym[x]];
+                    for (int x = 0; x < 8; ++x) b[x] = br[sym[x]];
 #pragma unroll
-                    for (int t = 0; t < CORR_MT; ++t) {
-                        if (mp + t < NMT) {
-                            const int i = (mp + t) * 32 + sl;
-                            const int u0 = (pb - 2 * i) & (N - 1);
-                            const uint32_t* q32 = reinterpret_cast<const uint32_t*>(qr + u0);
-                            union {
-                                uint32_t w[4];
-                                half8 v;
-                            } a;
-                            a.w[0] = q32[0];
-                            a.w[1] = q32[1];
-                            a.w[2] = q32[2];
-                            a.w[3] = q32[3];
-                            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(a.v, bfrag, acc[t], 0,
-                                                                            0, 0);
+                    for (int t = 0; t < MT; ++t) {
+                        const uint32_t* q32 =
+                            reinterpret_cast<const uint32_t*>(qr + ((ubase - 64 * t) & (N - 1)));
+                        union {
+                            uint32_t w[4];
+                            half8 v;
+                        } u;
+                        u.w[0] = q32[0];
+                        u.w[1] = q32[1];
+                        u.w[2] = q32[2];
+                        u.w[3] = q32[3];
+                        a[t] = u.v;
+                    }
+                };
+                load(0, bcur, acur);
+                for (int r = 0; r < k; ++r) {
+                    if (r + 1 < k) load(r + 1, bnxt, anxt);
+#pragma unroll
+                    for (int t = 0; t < MT; ++t)
+                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(acur[t], bcur, acc[t], 0,
+                                                                        0, 0);
+                    bcur = bnxt;
+#pragma unroll
+                    for (int t = 0; t < MT; ++t) acur[t] = anxt[t];
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < MT; ++t)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) best = fmaxf(best, acc[t][e]);
+        }
+        best = fmaxf(best, __shfl_xor(best, 32, 64));
+        if (h == 0 && in) out[sid] = best;
+    }
+}
+
+// Whether the A block whose lane-0 column starts at U (mod N) has a nonzero
+// entry.  Over the 64 lanes the block reads qtab[U-62 .. U+15], and qtab[u]
+// is nonzero only where (-u) mod N < Lq, i.e. for -u in the 78-value window
+// starting at (-U-15) mod N.
+__device__ inline bool band_nonzero(int U, int N, int Lq) {
+    if (Lq <= 0) return false;
+    if (N <= 78) return true;
+    const int v0 = (-U - 15) & (N - 1);
+    return v0 < Lq || v0 + 77 >= N;
+}
+
+// ---------------------------------------------------------------------------
+// Rank-4 database kernel with a register ring of A fragments.
+//
+// For m-tile t and 16-column chunk c the A fragment depends only on
+// 16c - 64t (mod N): the fragment of m-tile t at chunk c+4 is the one m-tile
+// t-1 used at chunk c.  Walking the chunks of one residue class mod 4 in
+// order therefore needs one new fragment per rank and step (m-tile 0's)
+// instead of MT; the other MT-1 rotate through a register ring whose slot
+// for m-tile t at unrolled step s is (t - s) mod MT.  B values of all four
+// ranks come from one 8-byte table entry per symbol.
+// Dynamic LDS: qtab 4*(N+8) fp16 | btab4 32 x 4 fp16 | stage CAPB+64 bytes.
+// ---------------------------------------------------------------------------
+template <int MT>
+__global__ __launch_bounds__(64) void k_corr_db4(const uint8_t* __restrict__ packed,
+                                                 const int64_t* __restrict__ off,
+                                                 const int32_t* __restrict__ len, int64_t s_begin,
+                                                 int64_t nseq, const uint8_t* __restrict__ qrev,
+                                                 int Lq, const float* __restrict__ lutA,
+                                                 const float* __restrict__ lutB, int V, int N,
+                                                 int maxlen, int CAPB, float* __restrict__ out) {
+    constexpr int K = 4;
+    extern __shared__ __align__(16) uint8_t smem[];
+    _Float16* qtab = reinterpret_cast<_Float16*>(smem);
+    const int QS = N + 8;
+    uint64_t* btab4 = reinterpret_cast<uint64_t*>(qtab + K * QS);  // [40] x 4 fp16
+    uint8_t* stg = reinterpret_cast<uint8_t*>(btab4 + CORR_NB);
+
+    const int lane = threadIdx.x;
+    const int Lqe = min(Lq, N);
+    for (int idx = lane; idx < K * QS; idx += 64) {
+        int r = idx / QS, u = idx - r * QS;
+        int t = (N - (u & (N - 1))) & (N - 1);
+        qtab[idx] = (_Float16)(t < Lqe ? 0.5f * lutA[r * V + (int)qrev[t]] : 0.f);
+    }
+    for (int c = lane; c < CORR_NB; c += 64) {
+        union {
+            _Float16 h[4];
+            uint64_t u;
+        } e;
+#pragma unroll
+        for (int r = 0; r < K; ++r)
+            e.h[r] = (_Float16)(c < 32 ? lutB[r * V + corr_code((uint32_t)c + 64u)] : 0.f);
+        btab4[c] = e.u;
+    }
+
+    const int sl = lane & 31, h = lane >> 5;
+    const int NMT = N >= 64 ? N / 64 : 1;
+    for (int64_t tile = blockIdx.x; tile * 32 < nseq; tile += gridDim.x) {
+        const int64_t sid = tile * 32 + sl;
+        const bool in = sid < nseq;
+        int64_t o = 0;
+        int Le = 0;
+        if (in) {
+            o = off[s_begin + sid];
+            Le = max(0, min(min(len[s_begin + sid], maxlen), N));
+        }
+        int kp = Le;
+        int64_t lo = in ? o : INT64_MAX, hi = in ? o : 0;
+#pragma unroll
+        for (int d = 16; d > 0; d >>= 1) {
+            kp = max(kp, __shfl_xor(kp, d, 64));
+            lo = min(lo, __shfl_xor(lo, d, 64));
+            hi = max(hi, __shfl_xor(hi, d, 64));
+        }
+        kp = (kp + 15) & ~15;
+        if (!in) o = lo;
+        const int64_t b_lo = ((lo * 5) >> 3) & ~(int64_t)15;
+        const int64_t b_hi = (((hi + kp) * 5) >> 3) + 24;
+        const bool fast = (b_hi - b_lo) <= CAPB;
+        __syncthreads();
+        if (fast) {
+            const uint4* src = reinterpret_cast<const uint4*>(packed + b_lo);
+            uint4* dst = reinterpret_cast<uint4*>(stg);
+            const int nchunk = (int)((b_hi - b_lo + 15) >> 4);
+            for (int c = lane; c < nchunk; c += 64) dst[c] = src[c];
+        }
+        __syncthreads();
+        const int64_t relbit = 5 * o - 8 * b_lo;
+        const int C = kp >> 4;  // chunks holding residues
+
+        // B fragments of the 4 ranks for chunk c (this lane: 8 columns)
+        auto bfrags = [&](int c, half8* b) {
+            const int pb = 16 * c + 8 * h;
+            uint32_t sym[8];
+            if (fast) {
+                const int64_t bit = relbit + 5 * (int64_t)pb;
+                const int64_t byte = bit >> 3;
+                const uint64_t* s64 = reinterpret_cast<const uint64_t*>(stg + (byte & ~7));
+                const uint64_t B0 = __builtin_bswap64(s64[0]);
+                const uint64_t B1 = __builtin_bswap64(s64[1]);
+                const int sh = (int)(byte & 7) * 8 + (int)(bit & 7);
+                const uint64_t top = sh ? (B0 << sh) | (B1 >> (64 - sh)) : B0;
+#pragma unroll
+                for (int x = 0; x < 8; ++x) sym[x] = (uint32_t)(top >> (59 - 5 * x)) & 31u;
+            } else {
+#pragma unroll
+                for (int x = 0; x < 8; ++x)
+                    sym[x] = pb + x < Le ? packed_sym(packed, o + pb + x) - 64u : 0u;
+            }
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+                const uint32_t ix = pb + x < Le ? sym[x] : (uint32_t)CORR_PAD;
+                union {
+                    uint64_t u;
+                    _Float16 h4[4];
+                } e;
+                e.u = btab4[ix];
+#pragma unroll
+                for (int r = 0; r < K; ++r) b[r][x] = e.h4[r];
+            }
+        };
+        // A fragment of rank r whose first column is u (mod N) for this lane
+        auto afrag = [&](int r, int u) -> half8 {
+            const uint32_t* q32 =
+                reinterpret_cast<const uint32_t*>(qtab + r * QS + (u & (N - 1)));
+            union {
+                uint32_t w[4];
+                half8 v;
+            } a;
+            a.w[0] = q32[0];
+            a.w[1] = q32[1];
+            a.w[2] = q32[2];
+            a.w[3] = q32[3];
+            return a.v;
+        };
+
+        float best = -INFINITY;
+        for (int mp = 0; mp < NMT; mp += MT) {
+            f32x16 acc[MT];
+#pragma unroll
+            for (int t = 0; t < MT; ++t)
+#pragma unroll
+                for (int e = 0; e < 16; ++e) acc[t][e] = 0.f;
+            // lane base: m-tile (mp + t), column 8h - 2*sl of the fragment
+            const int lb = 8 * h - 2 * (mp * 32 + sl);
+            for (int j = 0; j < 4 && j < C; ++j) {
+                half8 ring[K][MT];
+                uint32_t nzmask = 0;  // bit k: ring slot k holds a fragment with a nonzero
+#pragma unroll
+                for (int t = 0; t < MT; ++t) {
+#pragma unroll
+                    for (int r = 0; r < K; ++r) ring[r][t] = afrag(r, lb + 16 * j - 64 * t);
+                    nzmask |= (uint32_t)band_nonzero(16 * (j - 4 * (mp + t)), N, Lqe) << t;
+                }
+                for (int cg = j; cg < C; cg += 4 * MT) {
+#pragma unroll
+                    for (int s = 0; s < MT; ++s) {
+                        const int c = cg + 4 * s;
+                        if (c < C) {
+                            half8 b[K];
+                            bfrags(c, b);
+                            if (c != j) {
+                                const int slot = (MT - s) % MT;
+#pragma unroll
+                                for (int r = 0; r < K; ++r) ring[r][slot] = afrag(r, lb + 16 * c);
+                                nzmask = (nzmask & ~(1u << slot)) |
+                                         ((uint32_t)band_nonzero(16 * (c - 4 * mp), N, Lqe)
+                                          << slot);
+                            }
+                            // blocks whose A fragment is all zero add exact zeros: skip
+#pragma unroll
+                            for (int r = 0; r < K; ++r)
+#pragma unroll
+                                for (int t = 0; t < MT; ++t) {
+                                    const int slot = (t - s + MT) % MT;
+                                    if (nzmask & (1u << slot))
+                                        acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_f16(
+                                            ring[r][slot], b[r], acc[t], 0, 0, 0);
+                                }
                         }
                     }
                 }
             }
 #pragma unroll
-            for (int t = 0; t < CORR_MT; ++t) {
-                if (mp + t < NMT) {
+            for (int t = 0; t < MT; ++t)
 #pragma unroll
-                    for (int e = 0; e < 16; ++e) best = fmaxf(best, acc[t][e]);
-                }
-            }
+                for (int e = 0; e < 16; ++e) best = fmaxf(best, acc[t][e]);
         }
         best = fmaxf(best, __shfl_xor(best, 32, 64));
         if (h == 0 && in) out[sid] = best;
@@ -387,14 +590,51 @@ extern "C" int sseqs_db_prefilter(const uint8_t* packed, const int64_t* seq_off,
     size_t lds = (size_t)k * (N + 8) * 2 + (size_t)k * CORR_NB * 2 + (size_t)CAPB + 64;
     lds = (lds + 15) & ~(size_t)15;
     if (lds > 160 * 1024) return fail(-1, "db_prefilter: tables exceed LDS (reduce k or N)");
-    if (lds > 64 * 1024 &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_corr_db),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-        return fail(-3, "db_prefilter: cannot enable large LDS");
     const int64_t tiles = ceil_div(nseq, 32);
     const unsigned grid = (unsigned)(tiles < 8192 ? tiles : 8192);
-    hipLaunchKernelGGL(k_corr_db, dim3(grid), dim3(64), lds, stream, packed, seq_off, seq_len,
-                       s_begin, nseq, qrev, query_len, lutA, lutB, vocab_size, k, N, mlen, CAPB,
-                       scores);
-    return check_launch("db_prefilter");
+    int nmt = N >= 64 ? N / 64 : 1;
+    // m-tiles per pass: more reuse of each decoded B fragment against more
+    // registers (8 -> 1 wave per SIMD, 4 -> 2).  Measured on gfx950 with
+    // rank 4: N = 512 runs fastest at 4, N = 1024 and 2048 at 8.
+    if (nmt > 4 && N <= 512) nmt = 4;
+    // SSEQS_CORR_MT caps it further (tuning knob; results do not depend on it)
+    if (const char* e = getenv("SSEQS_CORR_MT")) {
+        int cap = atoi(e);
+        while (nmt > 1 && nmt > cap) nmt >>= 1;
+    }
+    auto go = [&](auto kern) -> int {
+        if (lds > 64 * 1024 &&
+            hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+            return fail(-3, "db_prefilter: cannot enable large LDS");
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, stream, packed, seq_off, seq_len,
+                           s_begin, nseq, qrev, query_len, lutA, lutB, vocab_size, k, N, mlen,
+                           CAPB, scores);
+        return check_launch("db_prefilter");
+    };
+    if (k == 4) {
+        auto go4 = [&](auto kern) -> int {
+            if (lds > 64 * 1024 &&
+                hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)lds) != hipSuccess)
+                return fail(-3, "db_prefilter: cannot enable large LDS");
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, stream, packed, seq_off, seq_len,
+                               s_begin, nseq, qrev, query_len, lutA, lutB, vocab_size, N, mlen,
+                               CAPB, scores);
+            return check_launch("db_prefilter");
+        };
+        switch (nmt) {
+        case 1: return go4(&k_corr_db4<1>);
+        case 2: return go4(&k_corr_db4<2>);
+        case 4: return go4(&k_corr_db4<4>);
+        default: return go4(&k_corr_db4<8>);
+        }
+    }
+    switch (nmt) {
+    case 1: return go(&k_corr_db<1>);
+    case 2: return go(&k_corr_db<2>);
+    case 4: return go(&k_corr_db<4>);
+    default: return go(&k_corr_db<8>);
+    }
 }

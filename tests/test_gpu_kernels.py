@@ -189,15 +189,15 @@ def test_warp_corr_matches_fp64(orc, dev, N, k):
     assert ok.all(), (N, k, np.max(np.abs(got - want)), got[~ok][:5], want[~ok][:5])
 
 
-@pytest.mark.parametrize("Lq", [50, 200, 500])
-def test_db_prefilter_any_index_order(orc, dev, Lq):
+@pytest.mark.parametrize("Lq,k", [(50, 4), (200, 4), (500, 4), (200, 1), (90, 3)])
+def test_db_prefilter_any_index_order(orc, dev, Lq, k):
     """Resident-database prefilter on an index that is length-sorted but not
     in stream order (a file that was not sorted), which takes the kernel's
     global-memory path, and on the same data in stream order."""
     from sseqs_amd.db import SeqDB
     from sseqs_amd.query import prepare
     from sseqs_amd.search import prefilter_scores
-    announce(f"db_prefilter unsorted Lq={Lq}")
+    announce(f"db_prefilter unsorted Lq={Lq} k={k}")
     rng = np.random.default_rng(Lq)
     query = rand_seq(rng, Lq)
     seqs = [mutate(rng, query) if i % 5 == 0 else rand_seq(rng, rng.integers(1, 1100))
@@ -205,14 +205,14 @@ def test_db_prefilter_any_index_order(orc, dev, Lq):
     text = ("@".join(seqs) + "@").encode()
     db = SeqDB.from_packed(_packed_db(dev, text), len(text))
     assert db.orig_id is not None
-    got = prefilter_scores(db, prepare(query)).cpu().numpy()
+    got = prefilter_scores(db, prepare(query, fft_rank=k)).cpu().numpy()
     order = db.orig_id.cpu().numpy()
-    A, B = orc.weighted_luts(query, 4)
+    A, B = orc.weighted_luts(query, k)
     qrev = orc.encode_query_rev(query)
     want = np.zeros(len(seqs))
     for i, s in enumerate(seqs):
         N = min(2048, orc.nextpow2(max(Lq, len(s), 16)))
-        want[i] = orc.corr(qrev, A, B, 4, s.encode(), [0], [len(s)], N, N)[0]
+        want[i] = orc.corr(qrev, A, B, k, s.encode(), [0], [len(s)], N, N)[0]
     want = want[order]
     assert corr_tol(got, want).all(), np.max(np.abs(got - want))
     # the same sequences already sorted: contiguous tiles (staged path)
@@ -220,7 +220,7 @@ def test_db_prefilter_any_index_order(orc, dev, Lq):
     text2 = ("@".join(srt) + "@").encode()
     db2 = SeqDB.from_packed(_packed_db(dev, text2), len(text2))
     assert db2.orig_id is None
-    got2 = prefilter_scores(db2, prepare(query)).cpu().numpy()
+    got2 = prefilter_scores(db2, prepare(query, fft_rank=k)).cpu().numpy()
     assert np.array_equal(got2, got)
 
 
