@@ -94,6 +94,10 @@ int launch_sw_cuda_affine(const uint8_t* query_seq_indices_ptr, int query_length
 
 /* ------------------------------------------------ HBM-resident database -- */
 
+/* A packed database buffer holds ceil(5*n_syms/8) bytes followed by at least
+ * 64 readable bytes: the kernels below read whole 16-byte words and 8-symbol
+ * windows, which may extend up to 40 bytes past a sequence's last residue. */
+
 /* Index of a packed database holding n_syms symbols: every maximal run of
  * non-'@' symbols is one sequence.  Writes up to `cap` entries of
  * seq_off (symbol offset of the first residue) and seq_len, and the number

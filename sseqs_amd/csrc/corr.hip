@@ -18,12 +18,15 @@
 // read.  B fragments are table lookups of the staged residue codes.  Inputs
 // are fp16, accumulation fp32, so there is no fp16 overflow (the reference's
 // all-fp16 evaluation saturates to inf on strong hits).
+#include <type_traits>
+
 #include "common.h"
 
 namespace sseqs {
 
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int CORR_PAD = 39;   // code index whose table value is 0
 constexpr int CORR_NB = 40;    // table entries per rank
@@ -50,6 +53,50 @@ struct CorrPacked {
     // table index: the 5-bit symbol itself (the table maps it through the code)
     __device__ uint32_t code(int64_t o, int p) const { return packed_sym(packed, o + p) - 64u; }
 };
+
+// Copies the packed bytes of one tile of database sequences into the LDS
+// stage.  lo / hi: smallest and largest symbol offset in the tile; ehi: end
+// (offset + residues used) of the sequence reaching furthest; span: symbols
+// past its offset that a lane addresses with 8-symbol windows (the ones
+// beyond its length are masked).  Returns whether the tile fits the stage
+// (else its symbols are read from global memory) and the stage's first
+// byte.  Only bytes holding residues in use are copied, so a tile never
+// reads more than 24 bytes past the last residue of the database.
+struct Stage {
+    int64_t b_lo;
+    bool fast;
+};
+__device__ inline Stage stage_tile(const uint8_t* __restrict__ packed, uint8_t* stg, int CAPB,
+                                   int64_t lo, int64_t hi, int64_t ehi, int span, int lane) {
+    Stage st;
+    st.b_lo = ((lo * 5) >> 3) & ~(int64_t)15;
+    // the last window of a lane starts at byte 5*(o+span-8)/8 and is 16 bytes
+    st.fast = ((((hi + span) * 5) >> 3) + 24 - st.b_lo) <= CAPB;
+    const int64_t b_end = ((ehi * 5) >> 3) + 8;
+    __syncthreads();  // the previous tile no longer reads the stage
+    if (st.fast) {
+        const uint4* src = reinterpret_cast<const uint4*>(packed + st.b_lo);
+        uint4* dst = reinterpret_cast<uint4*>(stg);
+        const int64_t nchunk = (b_end - st.b_lo + 15) >> 4;
+        for (int64_t c = lane; c < nchunk; c += 64) dst[c] = src[c];
+    }
+    __syncthreads();
+    return st;
+}
+
+// 8 symbols (5-bit, minus 64) of a sequence starting at bit `bit` of the
+// stage: two aligned 8-byte LDS words, byte-swapped to big-endian order.
+__device__ inline void stage_syms(const uint8_t* stg, int64_t bit, uint32_t* sym) {
+    const int64_t byte = bit >> 3;
+    const uint64_t* s64 = reinterpret_cast<const uint64_t*>(stg + (byte & ~7));
+    const uint64_t B0 = __builtin_bswap64(s64[0]);
+    const uint64_t B1 = __builtin_bswap64(s64[1]);
+    const int sh = (int)(byte & 7) * 8 + (int)(bit & 7);
+    // (B1 >> 1) >> (63 - sh) is B1 >> (64 - sh), and 0 for sh == 0
+    const uint64_t top = (B0 << sh) | ((B1 >> 1) >> (63 - sh));
+#pragma unroll
+    for (int x = 0; x < 8; ++x) sym[x] = (uint32_t)(top >> (59 - 5 * x)) & 31u;
+}
 
 template <class OutT>
 __device__ inline void store_score(OutT* out, int64_t i, float v);
@@ -233,30 +280,19 @@ __global__ __launch_bounds__(64) void k_corr_db(const uint8_t* __restrict__ pack
             Le = max(0, min(min(len[s_begin + sid], maxlen), N));
         }
         int kp = Le;
-        int64_t lo = in ? o : INT64_MAX, hi = in ? o : 0;
+        int64_t lo = in ? o : INT64_MAX, hi = in ? o : 0, ehi = in ? o + Le : 0;
 #pragma unroll
         for (int d = 16; d > 0; d >>= 1) {
             kp = max(kp, __shfl_xor(kp, d, 64));
             lo = min(lo, __shfl_xor(lo, d, 64));
             hi = max(hi, __shfl_xor(hi, d, 64));
+            ehi = max(ehi, __shfl_xor(ehi, d, 64));
         }
         kp = (kp + 15) & ~15;
         if (!in) o = lo;  // idle lanes decode (and mask) inside the stage
-        // Symbols read: [lo, hi + kp).  The last window of a lane starts at
-        // byte 5*(o+kp-8)/8 and spans 16 bytes, so 24 bytes past 5*(hi+kp)/8
-        // cover every read; the start is 16-byte aligned for the copy.
-        const int64_t b_lo = ((lo * 5) >> 3) & ~(int64_t)15;
-        const int64_t b_hi = (((hi + kp) * 5) >> 3) + 24;
-        const bool fast = (b_hi - b_lo) <= CAPB;
-        __syncthreads();  // the previous tile no longer reads the stage
-        if (fast) {
-            const uint4* src = reinterpret_cast<const uint4*>(packed + b_lo);
-            uint4* dst = reinterpret_cast<uint4*>(stg);
-            const int nchunk = (int)((b_hi - b_lo + 15) >> 4);
-            for (int c = lane; c < nchunk; c += 64) dst[c] = src[c];
-        }
-        __syncthreads();
-        const int64_t relbit = 5 * o - 8 * b_lo;  // bit of this sequence in the stage
+        const Stage st = stage_tile(packed, stg, CAPB, lo, hi, ehi, kp, lane);
+        const bool fast = st.fast;
+        const int64_t relbit = 5 * o - 8 * st.b_lo;  // bit of this sequence in the stage
 
         float best = -INFINITY;
         for (int mp = 0; mp < NMT; mp += MT) {
@@ -269,15 +305,7 @@ __global__ __launch_bounds__(64) void k_corr_db(const uint8_t* __restrict__ pack
                 const int pb = p0 + 8 * h;
                 uint32_t sym[8];
                 if (fast) {
-                    const int64_t bit = relbit + 5 * (int64_t)pb;
-                    const int64_t byte = bit >> 3;
-                    const uint64_t* s64 = reinterpret_cast<const uint64_t*>(stg + (byte & ~7));
-                    const uint64_t B0 = __builtin_bswap64(s64[0]);
-                    const uint64_t B1 = __builtin_bswap64(s64[1]);
-                    const int sh = (int)(byte & 7) * 8 + (int)(bit & 7);
-                    const uint64_t top = sh ? (B0 << sh) | (B1 >> (64 - sh)) : B0;
-#pragma unroll
-                    for (int x = 0; x < 8; ++x) sym[x] = (uint32_t)(top >> (59 - 5 * x)) & 31u;
+                    stage_syms(stg, relbit + 5 * (int64_t)pb, sym);
                 } else {
 #pragma unroll
                     for (int x = 0; x < 8; ++x)
@@ -400,27 +428,19 @@ __global__ __launch_bounds__(64) void k_corr_db4(const uint8_t* __restrict__ pac
             Le = max(0, min(min(len[s_begin + sid], maxlen), N));
         }
         int kp = Le;
-        int64_t lo = in ? o : INT64_MAX, hi = in ? o : 0;
+        int64_t lo = in ? o : INT64_MAX, hi = in ? o : 0, ehi = in ? o + Le : 0;
 #pragma unroll
         for (int d = 16; d > 0; d >>= 1) {
             kp = max(kp, __shfl_xor(kp, d, 64));
             lo = min(lo, __shfl_xor(lo, d, 64));
             hi = max(hi, __shfl_xor(hi, d, 64));
+            ehi = max(ehi, __shfl_xor(ehi, d, 64));
         }
         kp = (kp + 15) & ~15;
         if (!in) o = lo;
-        const int64_t b_lo = ((lo * 5) >> 3) & ~(int64_t)15;
-        const int64_t b_hi = (((hi + kp) * 5) >> 3) + 24;
-        const bool fast = (b_hi - b_lo) <= CAPB;
-        __syncthreads();
-        if (fast) {
-            const uint4* src = reinterpret_cast<const uint4*>(packed + b_lo);
-            uint4* dst = reinterpret_cast<uint4*>(stg);
-            const int nchunk = (int)((b_hi - b_lo + 15) >> 4);
-            for (int c = lane; c < nchunk; c += 64) dst[c] = src[c];
-        }
-        __syncthreads();
-        const int64_t relbit = 5 * o - 8 * b_lo;
+        const Stage st = stage_tile(packed, stg, CAPB, lo, hi, ehi, kp, lane);
+        const bool fast = st.fast;
+        const int64_t relbit = 5 * o - 8 * st.b_lo;
         const int C = kp >> 4;  // chunks holding residues
 
         // B fragments of the 4 ranks for chunk c (this lane: 8 columns)
@@ -428,15 +448,7 @@ __global__ __launch_bounds__(64) void k_corr_db4(const uint8_t* __restrict__ pac
             const int pb = 16 * c + 8 * h;
             uint32_t sym[8];
             if (fast) {
-                const int64_t bit = relbit + 5 * (int64_t)pb;
-                const int64_t byte = bit >> 3;
-                const uint64_t* s64 = reinterpret_cast<const uint64_t*>(stg + (byte & ~7));
-                const uint64_t B0 = __builtin_bswap64(s64[0]);
-                const uint64_t B1 = __builtin_bswap64(s64[1]);
-                const int sh = (int)(byte & 7) * 8 + (int)(bit & 7);
-                const uint64_t top = sh ? (B0 << sh) | (B1 >> (64 - sh)) : B0;
-#pragma unroll
-                for (int x = 0; x < 8; ++x) sym[x] = (uint32_t)(top >> (59 - 5 * x)) & 31u;
+                stage_syms(stg, relbit + 5 * (int64_t)pb, sym);
             } else {
 #pragma unroll
                 for (int x = 0; x < 8; ++x)
@@ -526,6 +538,289 @@ __global__ __launch_bounds__(64) void k_corr_db4(const uint8_t* __restrict__ pac
     }
 }
 
+// ---------------------------------------------------------------------------
+// Rank-4 band kernel on v_mfma_f32_16x16x32_f16.
+//
+// Rows are 16 even shifts (m-tile I: m = 2i, i in [16I, 16I+16)), columns 16
+// sequences, K = 32 positions (chunk c: p in [32c, 32c+32)).  The A block of
+// (I, c) depends only on the diagonal d = (I - c) mod ND, ND = N/32, and is
+// nonzero only for a circular run of Rn diagonals starting at d0, because
+// the query occupies Lq of the N correlation lags.  The A fragments of those
+// diagonals (all 4 ranks) stay in registers for the whole kernel, so each
+// chunk is decoded once and feeds Rn*4 MFMAs and nothing else.
+//
+// Accumulators form a ring over the R m-tiles a chunk can reach.  With
+// relative m-tile index rel = I - d0 (before reduction mod ND), chunk v
+// (taken in order) updates rel = v .. v+R-1 and rel = v gets its last update
+// at chunk v, so after that chunk it is folded into the running max and its
+// slot reused (slot = rel mod R).  rel >= ND is the circular wrap of the
+// correlation: shift tiles that the tail of a long sequence reaches again.
+// Their first x = C - ND + Rn - 1 counterparts (rel < x) are parked in LDS
+// when they close instead of being folded, and completed at the end.
+// m-tiles that no chunk reaches are exactly zero.  FULL: every diagonal is
+// nonzero (R == ND, d0 == 0); all m-tiles stay live and are folded at the end.
+//
+// Staging is asynchronous: while a tile is computed, the next tile's packed
+// bytes are copied global -> LDS (global_load_lds_dwordx4) into the second
+// stage buffer (DBL) and the index entries of the one after are loaded.
+//
+// Dynamic LDS: btab4 40 x 8 B | qtab 4*(N+8) fp16 during set-up, then
+// (not FULL) save (R-1) x 64 lanes x 16 B | 1 or 2 (DBL) stages of CAPB+64
+// bytes (rounded to 16).
+// ---------------------------------------------------------------------------
+__device__ inline float max4(f32x4 v) { return fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3])); }
+
+template <int R, bool FULL, bool DBL>
+__global__ __launch_bounds__(64, (R <= 8 ? 2 : 1)) void k_corr_band(const uint8_t* __restrict__ packed,
+                                                  const int64_t* __restrict__ off,
+                                                  const int32_t* __restrict__ len, int64_t s_begin,
+                                                  int64_t nseq, const uint8_t* __restrict__ qrev,
+                                                  int Lq, const float* __restrict__ lutA,
+                                                  const float* __restrict__ lutB, int V, int N,
+                                                  int maxlen, int CAPB, int d0, int Rn,
+                                                  float* __restrict__ out) {
+    constexpr int K = 4;
+    extern __shared__ __align__(16) uint8_t smem[];
+    uint64_t* btab4 = reinterpret_cast<uint64_t*>(smem);  // [40] x 4 fp16
+    uint8_t* region = smem + CORR_NB * 8;
+    _Float16* qtab = reinterpret_cast<_Float16*>(region);  // set-up only
+    f32x4* save = reinterpret_cast<f32x4*>(region);        // [R-1][64]
+    uint8_t* stg = region + (FULL ? 0 : (R - 1) * 64 * 16);
+
+    const int lane = threadIdx.x;
+    const int QS = N + 8;
+    const int Lqe = min(Lq, N);
+    for (int idx = lane; idx < K * QS; idx += 64) {
+        int r = idx / QS, u = idx - r * QS;
+        int t = (N - (u & (N - 1))) & (N - 1);
+        qtab[idx] = (_Float16)(t < Lqe ? 0.5f * lutA[r * V + (int)qrev[t]] : 0.f);
+    }
+    for (int c = lane; c < CORR_NB; c += 64) {
+        union {
+            _Float16 h[4];
+            uint64_t u;
+        } e;
+#pragma unroll
+        for (int r = 0; r < K; ++r)
+            e.h[r] = (_Float16)(c < 32 ? lutB[r * V + corr_code((uint32_t)c + 64u)] : 0.f);
+        btab4[c] = e.u;
+    }
+    __syncthreads();
+
+    // lane (col, g): A row col, B/C column col, positions 8g..8g+7 of a chunk
+    const int col = lane & 15, g = lane >> 4;
+    // A fragment of diagonal d0+j, rank r: T[16I+col][32c+8g+x] =
+    // qtab[(8g + x - 2*col - 32*(I-c)) mod N], x = 0..7
+    half8 A[R][K];
+#pragma unroll
+    for (int j = 0; j < R; ++j) {
+        const int u = (8 * g - 2 * col - 32 * (d0 + j)) & (N - 1);
+#pragma unroll
+        for (int r = 0; r < K; ++r) {
+            const uint32_t* q32 = reinterpret_cast<const uint32_t*>(qtab + r * QS + u);
+            union {
+                uint32_t w[4];
+                half8 v;
+            } a;
+            a.w[0] = q32[0];
+            a.w[1] = q32[1];
+            a.w[2] = q32[2];
+            a.w[3] = q32[3];
+            A[j][r] = a.v;
+        }
+    }
+    __syncthreads();  // the region now holds save + stage
+
+    const int ND = N >> 5;
+    const int64_t ntiles = (nseq + 15) >> 4;
+    int64_t tile = blockIdx.x;
+    if (tile >= ntiles) return;
+    // stage buffers: the next tile's bytes land in one (asynchronous
+    // global -> LDS copies) while the current tile is computed from the other
+    const int SB = (CAPB + 64 + 15) & ~15;
+    const int SB2 = DBL ? SB : 0;  // offset of the second stage buffer
+
+    // raw index entries of tile t's sequences (clamped into range, so the
+    // loads are unconditional and nothing waits for them before use)
+    auto meta = [&](int64_t t, int64_t& o, int& L) {
+        const int64_t sid = min(t * 16 + col, nseq - 1);
+        o = off[s_begin + sid];
+        L = len[s_begin + sid];
+    };
+    struct Tile {
+        int64_t o, b_lo, nchunk;
+        int Le, C;
+        bool in, fast;
+    };
+    auto info = [&](int64_t t, int64_t o, int L) -> Tile {
+        Tile T;
+        T.in = t * 16 + col < nseq;
+        const int Le = T.in ? max(0, min(min(L, maxlen), N)) : 0;
+        int kp = Le;
+        int64_t lo = T.in ? o : INT64_MAX, hi = T.in ? o : 0, ehi = T.in ? o + Le : 0;
+#pragma unroll
+        for (int d = 8; d > 0; d >>= 1) {
+            kp = max(kp, __shfl_xor(kp, d, 64));
+            lo = min(lo, __shfl_xor(lo, d, 64));
+            hi = max(hi, __shfl_xor(hi, d, 64));
+            ehi = max(ehi, __shfl_xor(ehi, d, 64));
+        }
+        T.C = __builtin_amdgcn_readfirstlane((kp + 31) >> 5);  // chunks holding residues, <= ND
+        T.o = T.in ? o : lo;  // idle lanes decode (and mask) inside the stage
+        T.Le = Le;
+        T.b_lo = ((lo * 5) >> 3) & ~(int64_t)15;
+        // the last window of a lane starts at byte 5*(o+32C-8)/8 and is 16 bytes
+        T.fast = ((((hi + 32 * T.C) * 5) >> 3) + 24 - T.b_lo) <= CAPB;
+        // copy only bytes holding residues in use: at most 24 bytes past the
+        // last residue of the database are read
+        T.nchunk = T.fast ? ((((ehi * 5) >> 3) + 8 - T.b_lo + 15) >> 4) : 0;
+        return T;
+    };
+    // One global_load_lds_dwordx4 copies 64 x 16 bytes: lane l's 16 bytes go
+    // to M0 + 16 l.  Issued from asm so that the compiler does not add waits
+    // for it before unrelated LDS reads; the loop waits for it explicitly.
+    auto issue = [&](const Tile& T, uint8_t* buf) {
+        const uint32_t lds0 =
+            (uint32_t)(uintptr_t)(__attribute__((address_space(3))) uint8_t*)buf;
+        for (int64_t c0 = 0; c0 < T.nchunk; c0 += 64)
+            if (c0 + lane < T.nchunk) {
+                const uint8_t* src = packed + T.b_lo + 16 * (c0 + lane);
+                asm volatile("global_load_lds_dwordx4 %1, off"
+                             :
+                             : "{m0}"(lds0 + 16u * (uint32_t)c0), "v"(src)
+                             : "memory");
+            }
+    };
+
+    int64_t o_n = 0, o_c = 0;
+    int Le_n = 0, Le_c = 0;
+    meta(tile, o_c, Le_c);
+    if (tile + gridDim.x < ntiles) meta(tile + gridDim.x, o_n, Le_n);
+    Tile cur = info(tile, o_c, Le_c);
+    issue(cur, stg);
+    int sb = 0;
+    int64_t prev_sid = -1;  // this lane's result of the previous tile, stored late
+    float prev_best = 0.f;
+    for (;;) {
+        // the current tile's bytes (and the next tile's index entries) are in
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        const int64_t tn = tile + gridDim.x;
+        Tile nxt{};
+        if (tn < ntiles) {
+            nxt = info(tn, o_n, Le_n);
+            if (DBL) issue(nxt, stg + (sb ^ 1) * SB2);
+        }
+        if (tn + gridDim.x < ntiles) meta(tn + gridDim.x, o_n, Le_n);
+        if (prev_sid >= 0) out[prev_sid] = prev_best;
+        const uint8_t* sbuf = stg + sb * SB2;
+        const int64_t o = cur.o;
+        const int Le = cur.Le, C = cur.C;
+        const int64_t relbit = 5 * o - 8 * cur.b_lo;
+
+        f32x4 acc[R];
+#pragma unroll
+        for (int t = 0; t < R; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        float best = -INFINITY;
+        const int x = FULL ? 0 : max(0, C - ND + Rn - 1);
+        for (int vb = 0; vb < C; vb += R) {
+#pragma unroll
+            for (int s = 0; s < R; ++s) {
+                const int v = vb + s;
+                if (v < C) {
+                    const int pb = 32 * v + 8 * g;
+                    uint32_t sym[8];
+                    if (cur.fast) {
+                        stage_syms(sbuf, relbit + 5 * (int64_t)pb, sym);
+                    } else {
+#pragma unroll
+                        for (int e = 0; e < 8; ++e)
+                            sym[e] = pb + e < Le ? packed_sym(packed, o + pb + e) - 64u : 0u;
+                    }
+                    half8 b[K];
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) {
+                        const uint32_t ix = pb + e < Le ? sym[e] : (uint32_t)CORR_PAD;
+                        union {
+                            uint64_t u;
+                            _Float16 h4[4];
+                        } w;
+                        w.u = btab4[ix];
+#pragma unroll
+                        for (int r = 0; r < K; ++r) b[r][e] = w.h4[r];
+                    }
+                    // consecutive MFMAs go to different accumulators
+#pragma unroll
+                    for (int r = 0; r < K; ++r)
+#pragma unroll
+                        for (int j = 0; j < R; ++j)
+                            acc[(s + j) % R] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                                A[j][r], b[r], acc[(s + j) % R], 0, 0, 0);
+                    if (!FULL) {
+                        // relative m-tile v is complete
+                        if (v < x)
+                            save[v * 64 + lane] = acc[s];
+                        else
+                            best = fmaxf(best, max4(acc[s]));
+                        acc[s] = f32x4{0.f, 0.f, 0.f, 0.f};
+                    }
+                }
+            }
+        }
+        if (FULL) {
+#pragma unroll
+            for (int t = 0; t < R; ++t) best = fmaxf(best, max4(acc[t]));
+        } else {
+            // live: rel = C .. C+R-2 in slots rel mod R
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                const int rel = C + ((t - C) % R + R) % R;
+                if (rel == C + R - 1) continue;
+                if (rel < ND)
+                    best = fmaxf(best, max4(acc[t]));
+                else if (rel - ND < x)
+                    best = fmaxf(best, max4(acc[t] + save[(rel - ND) * 64 + lane]));
+            }
+            if (C + R - 1 < ND) best = fmaxf(best, 0.f);  // m-tiles no chunk reaches
+        }
+        // the four lane groups hold different rows of the same columns
+        best = fmaxf(best, __shfl_xor(best, 16, 64));
+        best = fmaxf(best, __shfl_xor(best, 32, 64));
+        prev_best = best;
+        prev_sid = (g == 0 && cur.in) ? tile * 16 + col : -1;
+
+        tile = tn;
+        if (tile >= ntiles) break;
+        if (!DBL) {
+            __syncthreads();  // the stage is no longer read
+            issue(nxt, stg);
+        } else {
+            sb ^= 1;
+        }
+        cur = nxt;
+    }
+    if (prev_sid >= 0) out[prev_sid] = prev_best;
+}
+
+// Nonzero diagonals of the band kernel: diagonal d's A blocks read lags
+// (32d + e) mod N, e in [-31, 30], and the query fills lags [0, Lqe).  The
+// nonzero ones form one circular run; returns its start and length.
+static void band_diagonals(int N, int Lqe, int* d0, int* rn) {
+    const int ND = N / 32;
+    bool nz[64];
+    int cnt = 0;
+    for (int d = 0; d < ND; ++d) {
+        nz[d] = false;
+        for (int e = -31; e <= 30 && !nz[d]; ++e) nz[d] = ((32 * d + e) & (N - 1)) < Lqe;
+        cnt += nz[d];
+    }
+    *d0 = 0;
+    *rn = cnt;
+    if (cnt == 0 || cnt == ND) return;
+    for (int d = 0; d < ND; ++d)
+        if (nz[d] && !nz[(d + ND - 1) % ND]) *d0 = d;
+}
+
 template <class Src, class OutT, bool PACKED_CODES>
 static int launch_corr(const Src& src, int64_t s_begin, int64_t nseq, const uint8_t* qrev, int Lq,
                        const float* lutA, const float* lutB, int V, int k, int N, int maxlen,
@@ -612,6 +907,65 @@ extern "C" int sseqs_db_prefilter(const uint8_t* packed, const int64_t* seq_off,
                            CAPB, scores);
         return check_launch("db_prefilter");
     };
+    const int Lqe = query_len < N ? query_len : N;
+    if (k == 4 && N >= 128 && Lqe > 0 && !getenv("SSEQS_CORR_NO_BAND")) {
+        int d0 = 0, rn = 0;
+        band_diagonals(N, Lqe, &d0, &rn);
+        const int ND = N / 32;
+        const bool full = rn == ND;
+        int R = 0;
+        if (full) {
+            R = ND <= 8 ? ND : 0;
+        } else {
+            for (int c : {4, 6, 8, 10, 12})
+                if (!R && rn <= c) R = c;
+            // R > 8 runs at one wave per SIMD: worth it only for a narrow band
+            if (R > 8 && 2 * rn > ND) R = 0;
+        }
+        if (R) {
+            // a tile of 16 length-sorted sequences spans about 16 of them
+            const int span = (min(mlen, N) + 31) & ~31;
+            int capb = (int)(((int64_t)16 * (span + 1) * 5) / 8 + 48);
+            capb = (capb + 15) & ~15;
+            const size_t qt = (size_t)4 * (N + 8) * 2;
+            const size_t sbytes = (size_t)((capb + 64 + 15) & ~15);
+            const size_t save = full ? 0 : (size_t)(R - 1) * 64 * 16;
+            auto lds_for = [&](int nbuf) {
+                const size_t run = save + nbuf * sbytes;
+                return ((size_t)CORR_NB * 8 + (qt > run ? qt : run) + 15) & ~(size_t)15;
+            };
+            // two stage buffers while 8 blocks (2 waves per SIMD) still fit a CU
+            bool dbl = lds_for(2) <= 20 * 1024;
+            if (const char* e = getenv("SSEQS_CORR_DBL")) dbl = atoi(e) != 0;
+            const size_t blds = lds_for(dbl ? 2 : 1);
+            const int64_t btiles = ceil_div(nseq, 16);
+            const unsigned bgrid = (unsigned)(btiles < 8192 ? btiles : 8192);
+            auto gob = [&](auto kern) -> int {
+                if (blds > 64 * 1024 &&
+                    hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)blds) != hipSuccess)
+                    return fail(-3, "db_prefilter: cannot enable large LDS");
+                hipLaunchKernelGGL(kern, dim3(bgrid), dim3(64), blds, stream, packed, seq_off,
+                                   seq_len, s_begin, nseq, qrev, query_len, lutA, lutB,
+                                   vocab_size, N, mlen, capb, d0, rn, scores);
+                return check_launch("db_prefilter");
+            };
+            auto pick = [&](auto dbl_tag) -> int {
+                constexpr bool D = decltype(dbl_tag)::value;
+                if (full)
+                    return R == 4 ? gob(&k_corr_band<4, true, D>) : gob(&k_corr_band<8, true, D>);
+                switch (R) {
+                case 4: return gob(&k_corr_band<4, false, D>);
+                case 6: return gob(&k_corr_band<6, false, D>);
+                case 8: return gob(&k_corr_band<8, false, D>);
+                case 10: return gob(&k_corr_band<10, false, D>);
+                default: return gob(&k_corr_band<12, false, D>);
+                }
+            };
+            return dbl ? pick(std::true_type{}) : pick(std::false_type{});
+        }
+    }
     if (k == 4) {
         auto go4 = [&](auto kern) -> int {
             if (lds > 64 * 1024 &&
