@@ -1,0 +1,192 @@
+"""MSA server: the ColabFold/boltz ticket API and the JSON ``/msa`` route
+(mirror of server.py).
+
+Routes, as in server.py:30-110,133-169:
+
+* ``POST /ticket/msa`` and ``POST /ticket/pair``. The form field ``q`` holds
+  FASTA: header and sequence lines alternate. Sequences not yet in the cache
+  (``cache_msa/<CHUNKS>_<num>_<top_fft>_<rank>_<top_sw>_<top_sw_affine>/
+  <sha256>.a3m``) are searched with one ``msa()`` call. Each a3m gets '@'
+  replaced by 'X' and its first line replaced by the request's header; the
+  a3m files are joined with ``b"\\x00\\n"`` and packed as ``uniref.a3m``
+  (``pair.a3m`` on the pair route) plus an empty
+  ``bfd.mgnify30.metaeuk30.smag30.a3m`` into a tar.gz. The route answers
+  ``{"status": "COMPLETE", "id": <tar id>}``.
+* ``GET /result/download/{tar_id}``: that tar.gz, or 404.
+* ``POST /msa`` (JSON :class:`MSARequest`): one ndjson line,
+  ``{"type": "data_msa", "data": {i: a3m}, "num_msas": n, "cache_folder": f}``,
+  or ``{"error": ...}``.
+
+Differences from server.py:
+
+* The app is built by :func:`create_app`; ``main()`` serves it. The
+  reference starts uvicorn at import.
+* Searches run in a worker thread, one at a time, so the event loop keeps
+  serving downloads meanwhile. The reference called ``msa()`` inside the
+  event loop.
+* ``/msa`` passes the file names as ``filenames``, the name ``msa()``
+  takes. The reference passed ``filenames_a3m=`` (server.py:147), so its
+  ``/msa`` always answered with the error line.
+* The tar id is the hash of the last distinct sequence in request order.
+  The reference took the last one in Python ``set`` order.
+"""
+
+import argparse
+import hashlib
+import io
+import json
+import os
+import tarfile
+import threading
+from pathlib import Path
+from typing import Callable, List, Literal
+
+ENV_A3M = "bfd.mgnify30.metaeuk30.smag30.a3m"
+
+
+def _sha(s: str) -> str:
+    return hashlib.sha256(s.encode("utf-8")).hexdigest()
+
+
+def create_app(msa_fn: Callable | None = None, root: str | os.PathLike = ".",
+               chunks: int | None = None, msa_kwargs: dict | None = None):
+    """FastAPI app serving the reference routes.
+
+    msa_fn: the search, called as ``msa_fn(queries, filenames, **options)``;
+    default :func:`sseqs_amd.msa.msa`. root: directory holding ``cache_msa/``
+    and the tarballs. chunks: the database size tag of the cache folders
+    (default ``$CHUNKS`` or 4, server.py:25). msa_kwargs: extra keyword
+    arguments for every search (for example ``db=``).
+    """
+    from fastapi import FastAPI, HTTPException, Request
+    from fastapi.concurrency import run_in_threadpool
+    from fastapi.middleware.cors import CORSMiddleware
+    from fastapi.responses import FileResponse, Response, StreamingResponse
+    from pydantic import BaseModel, Field
+
+    if msa_fn is None:
+        from .msa import msa as msa_fn
+    root = Path(root)
+    chunks = int(os.environ.get("CHUNKS", 4)) if chunks is None else int(chunks)
+    extra = dict(msa_kwargs or {})
+    lock = threading.Lock()
+    result_store: dict[str, Path] = {}
+
+    app = FastAPI()
+    app.add_middleware(CORSMiddleware, allow_origins=["*"], allow_credentials=True,
+                       allow_methods=["*"], allow_headers=["*"])
+    app.state.result_store = result_store
+
+    def search(queries: list[str], filenames: list[str], **opts) -> None:
+        with lock:
+            msa_fn(queries, filenames, **opts, **extra)
+
+    async def form_q(request: Request) -> str:
+        """The form field q.  URL-encoded forms (what ColabFold/boltz clients
+        send) are parsed here; multipart needs python-multipart."""
+        ctype = request.headers.get("content-type", "")
+        if ctype.startswith("multipart/"):
+            fields = await request.form()
+            q = fields.get("q")
+        else:
+            from urllib.parse import parse_qs
+            q = parse_qs((await request.body()).decode("utf-8"), keep_blank_values=True).get("q")
+            q = q[0] if q else None
+        if q is None:
+            raise HTTPException(status_code=422, detail="form field 'q' is required")
+        return str(q)
+
+    @app.post("/ticket/msa", include_in_schema=False)
+    @app.post("/ticket/pair", include_in_schema=False)
+    async def ticket_msa(request: Request):
+        lines = (await form_q(request)).split("\n")
+        num = int(os.getenv("MSAS_NUM", 4096))
+        rank = int(os.getenv("MSAS_FFT_RANK", 4))
+        top_fft = int(os.getenv("MSAS_TOP_FFT", 200))
+        top_sw = int(os.getenv("MSAS_TOP_SW", 10))
+        top_sw_affine = int(os.getenv("MSAS_TOP_SW_AFFINE", 2))
+        folder = root / "cache_msa" / f"{chunks}_{num}_{top_fft}_{rank}_{top_sw}_{top_sw_affine}"
+        folder.mkdir(parents=True, exist_ok=True)
+
+        heads, proteins = lines[::2], lines[1::2]
+        out_paths = [folder / f"{_sha(p)}.a3m" for p in proteins]
+        distinct = list(dict.fromkeys(proteins))
+        todo = [p for p in distinct if not (folder / f"{_sha(p)}.a3m").exists()]
+        if todo:
+            await run_in_threadpool(search, todo, [str(folder / f"{_sha(p)}.a3m") for p in todo],
+                                    fft_rank=rank, top_fft=top_fft, top_sw=top_sw,
+                                    top_sw_affine=top_sw_affine, num_msas=num, bs=100_000_000)
+
+        parts = []
+        for i, path in enumerate(out_paths):
+            text = path.read_text().replace("@", "X")
+            rows = text.split("\n")
+            rows[0] = heads[i]
+            parts.append("\n".join(rows).encode())
+        content = b"\x00\n".join(parts)
+
+        tar_id = _sha(distinct[-1]) if distinct else _sha("")
+        tar_dir = root / f"{tar_id}_tar"
+        tar_dir.mkdir(parents=True, exist_ok=True)
+        tar_file = tar_dir / f"{tar_id}.tar.gz"
+        name = "pair.a3m" if request.url.path.endswith("/ticket/pair") else "uniref.a3m"
+        with tarfile.open(tar_file, "w:gz") as tar:
+            ti = tarfile.TarInfo(name=name)
+            ti.size = len(content)
+            tar.addfile(ti, io.BytesIO(content))
+            ti2 = tarfile.TarInfo(name=ENV_A3M)
+            ti2.size = 0
+            tar.addfile(ti2, io.BytesIO(b""))
+        result_store[tar_id] = tar_file
+        return {"status": "COMPLETE", "id": tar_id}
+
+    @app.get("/result/download/{tar_id}", include_in_schema=False)
+    async def result_download(tar_id: str):
+        path = result_store.get(tar_id)
+        if path is None or not path.exists():
+            return Response(status_code=404)
+        return FileResponse(path, media_type="application/gzip", filename="out.tar.gz")
+
+    class MSARequest(BaseModel):  # server.py:113-124
+        sequences: List[str] = Field(["ADAM", "ADA"])
+        msas_num: int = Field(4096)
+        fft_rank: int = Field(1)
+        top_fft: int = Field(10)
+        top_sw: int = Field(2)
+        top_sw_affine: int = Field(2)
+        dataset: Literal["uniref30_2302+bfd_mgy_colabfold"] = Field(...)
+
+    @app.post("/msa")
+    def get_msa(req: MSARequest):
+        folder = root / (f"{chunks}_{req.msas_num}_{req.top_fft}_{req.fft_rank}_{req.top_sw}_"
+                         f"{req.top_sw_affine}")
+        folder.mkdir(parents=True, exist_ok=True)
+        files = [str(folder / f"{_sha(p)}.a3m") for p in req.sequences]
+
+        def event_stream():
+            try:
+                search(list(req.sequences), files, num_msas=req.msas_num, bs=100_000_000,
+                       fft_rank=req.fft_rank, top_fft=req.top_fft, top_sw=req.top_sw,
+                       top_sw_affine=req.top_sw_affine)
+                data = {i: Path(f).read_text() for i, f in enumerate(files)}
+                yield json.dumps({"type": "data_msa", "data": data,
+                                  "num_msas": len(req.sequences), "cache_folder": folder.name})
+            except Exception as e:  # reported to the client, as server.py:160-161
+                yield json.dumps({"error": f"MSA did not succeed: {e}."})
+
+        return StreamingResponse(event_stream(), media_type="application/x-ndjson")
+
+    return app
+
+
+def main(argv=None) -> None:
+    ap = argparse.ArgumentParser(description="Run MSA Server")
+    ap.add_argument("-port", "-p", default=8000, type=int, help="Port for server. ")
+    args, _ = ap.parse_known_args(argv)
+    import uvicorn
+    uvicorn.run(create_app(), host="0.0.0.0", port=args.port, reload=False, access_log=False,
+                workers=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -144,3 +144,34 @@ def test_msa_batch_a3m(orc, world, tmp_path):
                      64)
     want = orc.a3m(query, [orc.backtrack(query, h.seq) for h in hits])
     assert out.read_text() == want
+
+
+def test_server_ticket_roundtrip(orc, world, tmp_path, monkeypatch):
+    """The ticket route on the real search: the tarball holds msa()'s a3m with
+    the request header and '@' -> 'X', as server.py:64-106."""
+    import io
+    import tarfile
+
+    pytest.importorskip("fastapi")
+    from fastapi.testclient import TestClient
+
+    import sseqs_amd
+    from sseqs_amd.server import create_app
+    announce("server ticket roundtrip")
+    db, query = world["db"], world["queries"][1]
+    for k, v in (("MSAS_NUM", "64"), ("MSAS_TOP_FFT", "20"), ("MSAS_TOP_SW", "4")):
+        monkeypatch.setenv(k, v)
+    client = TestClient(create_app(root=tmp_path, chunks=4,
+                                   msa_kwargs=dict(db=db, verbose=False)))
+    r = client.post("/ticket/msa", data={"q": f">101\n{query}"})
+    assert r.status_code == 200 and r.json()["status"] == "COMPLETE"
+    blob = client.get(f"/result/download/{r.json()['id']}").content
+    with tarfile.open(fileobj=io.BytesIO(blob), mode="r:gz") as tar:
+        got = tar.extractfile("uniref.a3m").read().decode()
+    direct = tmp_path / "direct.a3m"
+    sseqs_amd.msa([query], [str(direct)], num_msas=64, verbose=False, db=db, bs=100_000_000,
+                  top_fft=20, top_sw=4)
+    rows = direct.read_text().replace("@", "X").split("\n")
+    rows[0] = ">101"
+    assert got == "\n".join(rows)
+    assert got.count(">") > 10
