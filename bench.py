@@ -8,7 +8,9 @@ prefilter of every database sequence (low-rank correlation on MFMA), global
 top 100 000, affine Smith-Waterman (BLOSUM62, open 11 / extend 1) of those,
 top 4096, device backtrack of the 4096 hits (truncated to 500 residues).
 The metric's query length is 200 (``value``); the same cascade with the
-500-residue query is reported under ``q500``.
+500-residue query is reported under ``q500``, and repeated with linear-gap
+SW (-4 per gap position) in place of the affine stage under
+``q500_linear4``.
 
 Prints one JSON line.  ``python bench.py --gpus N`` with N > 1 exits: the
 build targets one GPU.
@@ -126,28 +128,29 @@ def main() -> int:
     scores_buf = torch.empty(db.n, dtype=torch.float32, device=dev)
     bt_ws = BacktrackWorkspace(args.top_hits, 512, 500, dev)
 
-    def step(q):
-        gh = global_search(db, q, args.top_prefilter, args.top_hits, scores_buf=scores_buf)
+    def step(q, linear_gap=None):
+        gh = global_search(db, q, args.top_prefilter, args.top_hits, scores_buf=scores_buf,
+                           linear_gap=linear_gap)
         al = backtrack_hits(db, q, gh.ids, bt_ws)
         return gh, al
 
-    def measure(query: str):
+    def measure(query: str, linear_gap=None):
         q = prepare(query, device=dev)
         for i in range(args.warmup):
-            step(q)
+            step(q, linear_gap)
             log(f"warmup {i + 1}/{args.warmup} q_len={len(query)}")
         torch.cuda.synchronize()
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         t_wall = time.perf_counter()
         for _ in range(args.steps):
-            gh, al = step(q)
+            gh, al = step(q, linear_gap)
         e1.record()
         torch.cuda.synchronize()
         t_wall = time.perf_counter() - t_wall
         ms = e0.elapsed_time(e1) / args.steps
         # stage breakdown with events on the same stream (one extra pass)
-        from sseqs_amd.search import prefilter_scores, sw_affine_ids
+        from sseqs_amd.search import prefilter_scores, sw_affine_ids, sw_linear_ids
         from sseqs_amd import ops
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
         ev[0].record()
@@ -155,7 +158,8 @@ def main() -> int:
         ev[1].record()
         cand = ops.topk_indices(pre, min(args.top_prefilter, db.n), sorted=False)
         ev[2].record()
-        cs = sw_affine_ids(db, q, cand)
+        cs = sw_affine_ids(db, q, cand) if linear_gap is None else sw_linear_ids(db, q, cand,
+                                                                                 linear_gap)
         ev[3].record()
         top = ops.topk_indices(cs, min(args.top_hits, cand.numel()), sorted=True)
         backtrack_hits(db, q, cand[top], bt_ws)
@@ -167,12 +171,13 @@ def main() -> int:
         flops = fft_equiv_flops(cls.length, cls.count, len(query), k)
         res = {
             "q_len": len(query),
+            "sw": "affine 11/1" if linear_gap is None else f"linear {linear_gap}",
             "ms_per_step": round(ms, 3),
             "wall_ms_per_step": round(1e3 * t_wall / args.steps, 3),
             "search_gcups": round(len(query) * db.residues / (ms / 1e3) / 1e9, 1),
             "db_gb_per_s": round(db.nbytes / (ms / 1e3) / 1e9, 1),
             "stages_ms": {"prefilter": round(st[0], 3), "select_100k": round(st[1], 3),
-                          "sw_affine": round(st[2], 3), "select_4096+backtrack": round(st[3], 3)},
+                          "sw": round(st[2], 3), "select_4096+backtrack": round(st[3], 3)},
             "sw_gcups": round(ncell / (st[2] / 1e3) / 1e9, 1),
             "prefilter_tflops_fft_equiv": round(flops / (st[0] / 1e3) / 1e12, 2),
             "prefilter_db_gb_per_s": round(db.nbytes / (st[0] / 1e3) / 1e9, 1),
@@ -182,9 +187,10 @@ def main() -> int:
         return res, flops, st[0]
 
     r200, flops200, pre_ms = measure(q200)
-    r500 = None
+    r500 = r500l = None
     if not args.no_q500:
         r500, _, _ = measure(q500)
+        r500l, _, _ = measure(q500, linear_gap=-4)
     cpu = None
     if not args.no_cpu:
         t0 = time.time()
@@ -225,6 +231,7 @@ def main() -> int:
                          "peak": HBM_PEAK_GBPS, "unit": "GB/s",
                          "frac": round(r200["prefilter_db_gb_per_s"] / HBM_PEAK_GBPS, 4)},
         "q500": r500,
+        "q500_linear4": r500l,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

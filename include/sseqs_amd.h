@@ -137,6 +137,14 @@ int sseqs_db_sw_affine(const uint8_t* packed, const int64_t* seq_off, const int3
                        const int64_t* cand, int64_t n, const uint8_t* query, int query_len,
                        int gap_open, int gap_extend, int32_t* out, hipStream_t stream);
 
+/* Linear-gap counterpart (gap_val <= 0 per gap position, reference stripe
+ * rule of launch_sw_cuda_blosum62) of sseqs_db_sw_affine.  Database-resident
+ * form of sw.sw_cuda_blosum62 (sseqs/sw.cu:55-179), for the linear-gap run
+ * of the at-scale cascade. */
+int sseqs_db_sw_linear(const uint8_t* packed, const int64_t* seq_off, const int32_t* seq_len,
+                       const int64_t* cand, int64_t n, const uint8_t* query, int query_len,
+                       int gap_val, int32_t* out, hipStream_t stream);
+
 /* ---------------------------------------------------------------- top-k -- */
 
 /* dtype codes for sseqs_topk */

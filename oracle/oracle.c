@@ -241,6 +241,17 @@ void orc_sw_affine_segments(const uint8_t* q, int Lq, const uint8_t* ascii, cons
     }
 }
 
+/* Linear-gap counterpart over exact DB segments (sseqs/sw.cu:55-179); the
+ * linear-gap run of the whole-DB cascade. */
+void orc_sw_linear_segments(const uint8_t* q, int Lq, const uint8_t* ascii, const int64_t* off,
+                            const int32_t* len, int64_t n, int gap, int32_t* out) {
+    #pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t g = 0; g < n; ++g) {
+        if (Lq == 0 || len[g] <= 0) { out[g] = -1; continue; }
+        out[g] = orc_sw_linear_one(q, Lq, ascii + off[g], len[g], gap);
+    }
+}
+
 /* Textbook Gotoh local alignment with the same recurrences but no stripe
  * rule (horizontal gaps may cross every column).  Not a reference function:
  * used only to pin the stripe-rule oracle (stripe <= textbook, and equal when

@@ -75,6 +75,7 @@ def lib() -> C.CDLL:
         L.orc_sw_linear.argtypes = [vp, i32, vp, i32, vp, vp, i32, vp]
         L.orc_sw_affine.argtypes = [vp, i32, vp, i32, vp, vp, i32, i32, vp]
         L.orc_sw_affine_segments.argtypes = [vp, i32, vp, vp, vp, i64, i32, i32, vp]
+        L.orc_sw_linear_segments.argtypes = [vp, i32, vp, vp, vp, i64, i32, vp]
         L.orc_corr.argtypes = [vp, i32, vp, vp, i32, i32, i32, vp, vp, vp, i32, i32, i32, vp]
         L.orc_backtrack.argtypes = [vp, i32, vp, i32, i32, i32, vp, vp, vp]
         L.orc_gen_db.argtypes = [vp, vp, vp, i32, vp, vp, vp, i32, i64, vp, vp]
@@ -210,6 +211,16 @@ def sw_affine_segments(q_codes, ascii, off, ln, o=11, e=1) -> np.ndarray:
     ln = np.ascontiguousarray(ln, np.int32)
     out = np.zeros(off.size, dtype=np.int32)
     lib().orc_sw_affine_segments(_p(q), q.size, _p(a), _p(off), _p(ln), off.size, o, e, _p(out))
+    return out
+
+
+def sw_linear_segments(q_codes, ascii, off, ln, gap=-4) -> np.ndarray:
+    """sseqs/sw.cu:55-179 over exact segments (offset, length)."""
+    q, a = _u8(q_codes), _u8(ascii)
+    off = np.ascontiguousarray(off, np.int64)
+    ln = np.ascontiguousarray(ln, np.int32)
+    out = np.zeros(off.size, dtype=np.int32)
+    lib().orc_sw_linear_segments(_p(q), q.size, _p(a), _p(off), _p(ln), off.size, gap, _p(out))
     return out
 
 

@@ -238,3 +238,15 @@ extern "C" int sseqs_db_sw_affine(const uint8_t* packed, const int64_t* seq_off,
     SrcPacked src{packed, seq_off, seq_len, cand};
     return launch_sw<true>(src, query, Lq, n, gap_open, gap_extend, out, stream);
 }
+
+extern "C" int sseqs_db_sw_linear(const uint8_t* packed, const int64_t* seq_off,
+                                  const int32_t* seq_len, const int64_t* cand, int64_t n,
+                                  const uint8_t* query, int Lq, int gap_val, int32_t* out,
+                                  hipStream_t stream) {
+    if (n < 0) return fail(-1, "db_sw_linear: negative count");
+    if (n == 0) return 0;
+    if (Lq < 1 || Lq > 1024) return fail(-1, "db_sw_linear: query length must be 1..1024");
+    if (gap_val > 0) return fail(-1, "db_sw_linear: gap_val must be <= 0");
+    SrcPacked src{packed, seq_off, seq_len, cand};
+    return launch_sw<false>(src, query, Lq, n, gap_val, 0, out, stream);
+}

@@ -68,9 +68,25 @@ def sw_affine_ids(db: SeqDB, q: PreparedQuery, ids: torch.Tensor, gap_open: int 
     return out
 
 
+def sw_linear_ids(db: SeqDB, q: PreparedQuery, ids: torch.Tensor, gap: int = -4) -> torch.Tensor:
+    """Linear-gap SW score (``gap`` per gap position, reference stripe rule)
+    of database sequences ``ids``."""
+    ids = ids.to(torch.int64).contiguous()
+    out = torch.empty(ids.numel(), dtype=torch.int32, device=db.device)
+    if ids.numel():
+        call("sseqs_db_sw_linear", ptr(db.packed), ptr(db.seq_off), ptr(db.seq_len), ptr(ids),
+             ids.numel(), ptr(q.codes), int(q.codes.numel()), int(gap), ptr(out),
+             stream_of(db.device))
+    return out
+
+
 def global_search(db: SeqDB, q: PreparedQuery, top_prefilter: int = 100_000,
                   top_hits: int = 4096, gap_open: int = 11, gap_extend: int = 1,
-                  scores_buf: torch.Tensor | None = None) -> GlobalHits:
+                  scores_buf: torch.Tensor | None = None,
+                  linear_gap: int | None = None) -> GlobalHits:
+    """The whole-database cascade.  With ``linear_gap`` set, the candidates
+    are scored with linear-gap SW (that penalty per gap position) instead of
+    affine SW."""
     if q.codes.numel() == 0:
         raise ValueError("query has no residues")
     if q.codes.numel() > 1024:
@@ -78,7 +94,10 @@ def global_search(db: SeqDB, q: PreparedQuery, top_prefilter: int = 100_000,
     pre = prefilter_scores(db, q, scores_buf)
     kc = min(int(top_prefilter), db.n)
     cand = ops.topk_indices(pre, kc, sorted=False)
-    cs = sw_affine_ids(db, q, cand, gap_open, gap_extend)
+    if linear_gap is None:
+        cs = sw_affine_ids(db, q, cand, gap_open, gap_extend)
+    else:
+        cs = sw_linear_ids(db, q, cand, linear_gap)
     kh = min(int(top_hits), kc)
     top = ops.topk_indices(cs, kh, sorted=True)
     return GlobalHits(cand[top], cs[top], pre, cand, cs)

@@ -75,6 +75,25 @@ def test_global_cascade(orc, world, qi):
     assert np.mean(hq[cand[top[:50]]] == qi) > 0.9
 
 
+def test_global_cascade_linear_gap(orc, world):
+    """The linear-gap (-4) run of the whole-database cascade: the same
+    candidates, scored by linear SW bit-exactly."""
+    from sseqs_amd.query import prepare
+    from sseqs_amd.search import global_search
+    announce("global cascade linear -4")
+    db, query = world["db"], world["queries"][0]
+    q = prepare(query)
+    ga = global_search(db, q, top_prefilter=3000, top_hits=500)
+    gl = global_search(db, q, top_prefilter=3000, top_hits=500, linear_gap=-4)
+    cand = gl.candidates.cpu().numpy()
+    assert np.array_equal(cand, ga.candidates.cpu().numpy())
+    qc = orc.encode_query_sw(query)
+    sw = orc.sw_linear_segments(qc, world["ascii"], world["off"][cand], world["ln"][cand], -4)
+    assert np.array_equal(gl.cand_scores.cpu().numpy(), sw)
+    top = orc.topk_desc(sw, 500)
+    assert np.array_equal(gl.ids.cpu().numpy(), cand[top])
+
+
 def test_batch_cascade(orc, world):
     from sseqs_amd.query import prepare
     from sseqs_amd.search import batch_search
