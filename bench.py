@@ -37,19 +37,7 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def fft_equiv_flops(Ls: np.ndarray, cnt: np.ndarray, Lq: int, k: int) -> float:
-    """Algorithmic prefilter flops, k*(10*H*log2 H + 20*H) per sequence with
-    H = N/2 and N = nextpow2(max(Lq, L)) as this build uses; (Ls, cnt) are
-    the distinct lengths and their counts."""
-    tot = 0.0
-    for L, c in zip(Ls, cnt):
-        N = 1 << (max(Lq, int(L), 16) - 1).bit_length()
-        H = N // 2
-        tot += c * k * (10 * H * np.log2(H) + 20 * H)
-    return float(tot)
-
-
-def cpu_baseline(db, query: str, target_s: float = 8.0) -> dict:
+def cpu_baseline(db, query: str, target_s: float = 15.0) -> dict:
     """The oracle's FP64 prefilter + stripe-rule affine SW (OpenMP C) on a
     sample of the same database, on this host's cores."""
     import torch
@@ -65,11 +53,17 @@ def cpu_baseline(db, query: str, target_s: float = 8.0) -> dict:
     def run(nsamp):
         ids = np.sort(rng.choice(db.n, size=nsamp, replace=False))
         rows, ln = db.gather(torch.from_numpy(ids).to(db.device), 1024)
-        rows, ln = rows.cpu().numpy(), ln.cpu().numpy()
-        flat = rows.reshape(-1)
-        off = (np.arange(nsamp, dtype=np.int64) * 1024)
+        # sequences back to back, so offsets stay far below 2^31 (the
+        # oracle's prefilter takes int32 offsets, like launch_warp_corr)
+        keep = torch.arange(rows.shape[1], device=rows.device)[None, :] < ln[:, None]
+        flat = rows[keep].cpu().numpy()
+        ln = ln.cpu().numpy()
+        off = np.concatenate([[0], np.cumsum(ln.astype(np.int64))[:-1]])
+        if flat.size >= 2**31:
+            raise ValueError("CPU baseline sample too large for int32 offsets")
         t0 = time.perf_counter()
-        Ns = np.array([1 << (max(Lq, int(L), 16) - 1).bit_length() for L in ln])
+        m = np.maximum(np.maximum(ln.astype(np.int64), Lq), 16)
+        Ns = np.left_shift(1, np.ceil(np.log2(m)).astype(np.int64))
         pre = np.zeros(nsamp)
         for N in np.unique(Ns):
             sel = np.nonzero(Ns == N)[0]
@@ -81,14 +75,43 @@ def cpu_baseline(db, query: str, target_s: float = 8.0) -> dict:
         dt = time.perf_counter() - t0
         return dt, int(ln.sum())
 
-    dt, res = run(2000)
-    nsamp = int(min(db.n, max(2000, 2000 * target_s / max(dt, 1e-3))))
+    nsamp = 2000
     dt, res = run(nsamp)
+    for _ in range(2):  # small samples overstate the per-sequence cost
+        nsamp = int(min(db.n, 4_000_000, max(2000, nsamp * target_s / max(dt, 1e-3))))
+        dt, res = run(nsamp)
+        if dt > 0.5 * target_s:
+            break
     return {"value": round(Lq * res / dt / 1e9, 3), "unit": "GCUPS", "cores": threads,
             "kind": "port",
             "sample": f"{nsamp} random sequences ({res} residues) of the same database, "
                       f"q_len={Lq}: FP64 prefilter of all + affine SW of the top 1/2000, "
                       f"{dt:.1f} s"}
+
+
+def copy_probe(nbytes: int, reps: int) -> None:
+    """A plain device copy of known size, for calibrating rocprofv3's
+    FETCH_SIZE / WRITE_SIZE counters against true bytes moved."""
+    import torch
+    dev = torch.device("cuda:0")
+    src = torch.ones(nbytes, dtype=torch.uint8, device=dev)
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
+
+
+def load_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc and
+    committed under profiles/ (None when no measurement is there)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            rec = json.load(fh).get(f"{kernel} | {workload}")
+    except (OSError, ValueError):
+        return None
+    return rec
 
 
 def main() -> int:
@@ -101,19 +124,27 @@ def main() -> int:
     ap.add_argument("--top-hits", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-q500", action="store_true")
+    ap.add_argument("--prefilter-only", action="store_true",
+                    help="run only the q_len=200 prefilter sweep --steps times (for counters)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="run a plain device copy of BYTES bytes and exit (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
               "(the database is replicated per GPU and queries are independent, "
               "so there is no multi-GPU path to measure)", file=sys.stderr)
         return 1
+    if args.copy_probe:
+        copy_probe(args.copy_probe, max(1, args.steps))
+        return 0
 
     import torch
 
     from sseqs_amd import synth
     from sseqs_amd.db import SeqDB
     from sseqs_amd.query import prepare
-    from sseqs_amd.search import BacktrackWorkspace, backtrack_hits, global_search
+    from sseqs_amd.search import (BacktrackWorkspace, backtrack_hits, global_search,
+                                  prefilter_kernel_name, prefilter_scores)
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
@@ -124,34 +155,70 @@ def main() -> int:
     torch.cuda.synchronize()
     log(f"generated {db.n} sequences, {db.residues} residues, {db.nbytes / 1e9:.2f} GB packed "
         f"in {time.time() - t0:.1f} s")
-    cls = db.extra["classes"]
     scores_buf = torch.empty(db.n, dtype=torch.float32, device=dev)
-    bt_ws = BacktrackWorkspace(args.top_hits, 512, 500, dev)
 
-    def step(q, linear_gap=None):
+    if args.prefilter_only:
+        q = prepare(q200, device=dev)
+        for i in range(args.steps):
+            prefilter_scores(db, q, scores_buf)
+            torch.cuda.synchronize()
+            log(f"prefilter sweep {i + 1}/{args.steps}")
+        return 0
+
+    bt_ws = BacktrackWorkspace(args.top_hits, 512, 500, dev)
+    seq_len64 = db.seq_len.to(torch.int64)
+
+    def range_info(q, N, b, e):
+        """Algorithmic units of one prefilter launch (length range [b, e))."""
+        name = prefilter_kernel_name(q.length, q.k, N)
+        n = e - b
+        H = N // 2
+        fft_flops = n * q.k * (10 * H * np.log2(H) + 20 * H)
+        L = seq_len64[b:e]
+        packed_bytes = 5 * int((L + 1).sum().item()) / 8
+        exec_flops = None
+        if name.startswith("k_corr_band<"):
+            # per 16-sequence tile: ceil(L/32) chunks x R diagonals x 4 ranks
+            # MFMAs of 16x16x32 (16384 flops each)
+            R = int(name[len("k_corr_band<"):].split(",")[0])
+            exec_flops = int(((L + 31) // 32).sum().item()) * R * 4 * 16384 / 16
+        return {"N": N, "sequences": n, "kernel": name, "fft_equiv_flops": fft_flops,
+                "packed_bytes": packed_bytes, "mfma_exec_flops": exec_flops}
+
+    def step(q, linear_gap=None, events=None):
         gh = global_search(db, q, args.top_prefilter, args.top_hits, scores_buf=scores_buf,
-                           linear_gap=linear_gap)
+                           linear_gap=linear_gap, events=events)
         al = backtrack_hits(db, q, gh.ids, bt_ws)
         return gh, al
 
     def measure(query: str, linear_gap=None):
         q = prepare(query, device=dev)
+        ranges = db.length_ranges(q.length)
         for i in range(args.warmup):
             step(q, linear_gap)
             log(f"warmup {i + 1}/{args.warmup} q_len={len(query)}")
         torch.cuda.synchronize()
+        # every prefilter launch is bracketed by events on its own stream
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(len(ranges) + 1)]
+               for _ in range(args.steps)]
         e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
         e0.record()
         t_wall = time.perf_counter()
-        for _ in range(args.steps):
-            gh, al = step(q, linear_gap)
+        for i in range(args.steps):
+            gh, al = step(q, linear_gap, evs[i])
         e1.record()
         torch.cuda.synchronize()
         t_wall = time.perf_counter() - t_wall
         ms = e0.elapsed_time(e1) / args.steps
+        launches = []
+        for j, (N, b, e) in enumerate(ranges):
+            info = range_info(q, N, b, e)
+            info["ms"] = float(np.mean([ev[j].elapsed_time(ev[j + 1]) for ev in evs]))
+            launches.append(info)
+        pre_ms = sum(x["ms"] for x in launches)
         # stage breakdown with events on the same stream (one extra pass)
-        from sseqs_amd.search import prefilter_scores, sw_affine_ids, sw_linear_ids
         from sseqs_amd import ops
+        from sseqs_amd.search import sw_affine_ids, sw_linear_ids
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
         ev[0].record()
         pre = prefilter_scores(db, q, scores_buf)
@@ -167,8 +234,7 @@ def main() -> int:
         torch.cuda.synchronize()
         st = [ev[i].elapsed_time(ev[i + 1]) for i in range(4)]
         ncell = int(db.seq_len[cand].to(torch.int64).sum().item()) * int(q.codes.numel())
-        k = q.k
-        flops = fft_equiv_flops(cls.length, cls.count, len(query), k)
+        flops = sum(x["fft_equiv_flops"] for x in launches)
         res = {
             "q_len": len(query),
             "sw": "affine 11/1" if linear_gap is None else f"linear {linear_gap}",
@@ -176,28 +242,41 @@ def main() -> int:
             "wall_ms_per_step": round(1e3 * t_wall / args.steps, 3),
             "search_gcups": round(len(query) * db.residues / (ms / 1e3) / 1e9, 1),
             "db_gb_per_s": round(db.nbytes / (ms / 1e3) / 1e9, 1),
-            "stages_ms": {"prefilter": round(st[0], 3), "select_100k": round(st[1], 3),
+            "stages_ms": {"prefilter": round(pre_ms, 3), "select_100k": round(st[1], 3),
                           "sw": round(st[2], 3), "select_4096+backtrack": round(st[3], 3)},
             "sw_gcups": round(ncell / (st[2] / 1e3) / 1e9, 1),
-            "prefilter_tflops_fft_equiv": round(flops / (st[0] / 1e3) / 1e12, 2),
-            "prefilter_db_gb_per_s": round(db.nbytes / (st[0] / 1e3) / 1e9, 1),
+            "prefilter_tflops_fft_equiv": round(flops / (pre_ms / 1e3) / 1e12, 2),
+            "prefilter_db_gb_per_s": round(db.nbytes / (pre_ms / 1e3) / 1e9, 1),
+            "prefilter_launches": [
+                {"N": x["N"], "sequences": x["sequences"], "kernel": x["kernel"],
+                 "ms": round(x["ms"], 3),
+                 "fft_equiv_tflops": round(x["fft_equiv_flops"] / x["ms"] / 1e9, 1),
+                 "mfma_exec_tflops": (round(x["mfma_exec_flops"] / x["ms"] / 1e9, 1)
+                                      if x["mfma_exec_flops"] else None),
+                 "gb_per_s": round(x["packed_bytes"] / x["ms"] / 1e6, 1)} for x in launches],
             "best_score": int(gh.scores[0].item()),
         }
         log(json.dumps(res))
-        return res, flops, st[0]
+        return res, launches
 
-    r200, flops200, pre_ms = measure(q200)
+    r200, launches200 = measure(q200)
     r500 = r500l = None
     if not args.no_q500:
-        r500, _, _ = measure(q500)
-        r500l, _, _ = measure(q500, linear_gap=-4)
+        r500, _ = measure(q500)
+        r500l, _ = measure(q500, linear_gap=-4)
     cpu = None
     if not args.no_cpu:
         t0 = time.time()
         cpu = cpu_baseline(db, q200)
         log(f"cpu baseline {cpu} in {time.time() - t0:.1f} s")
 
-    ach = flops200 / (pre_ms / 1e3) / 1e12
+    # roofline of the dominant kernel launch of the q_len=200 step
+    dom = max(launches200, key=lambda x: x["ms"])
+    workload = (f"q_len=200, N={dom['N']}, {dom['sequences']} sequences of the "
+                f"{db.n}-sequence database")
+    ach = dom["fft_equiv_flops"] / dom["ms"] / 1e9
+    gbs = dom["packed_bytes"] / dom["ms"] / 1e6
+    traffic = load_traffic(dom["kernel"], workload)
     out = {
         "metric": METRIC,
         "value": r200["search_gcups"],
@@ -222,14 +301,21 @@ def main() -> int:
         "db_gb_per_s": r200["db_gb_per_s"],
         "sw_gcups": r200["sw_gcups"],
         "stages_ms": r200["stages_ms"],
+        "prefilter_launches": r200["prefilter_launches"],
         "roofline": {"bound": "mfma", "achieved": round(ach, 2),
                      "peak": MFMA_F16_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(ach / MFMA_F16_PEAK_TFLOPS, 4), "traffic": None,
-                     "kernel": "k_corr (prefilter), FFT-equivalent algorithmic flops per sweep / "
-                               "summed launch time per sweep"},
-        "roofline_hbm": {"bound": "hbm", "achieved": r200["prefilter_db_gb_per_s"],
-                         "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                         "frac": round(r200["prefilter_db_gb_per_s"] / HBM_PEAK_GBPS, 4)},
+                     "frac": round(ach / MFMA_F16_PEAK_TFLOPS, 4),
+                     "traffic": traffic,
+                     "kernel": dom["kernel"], "launch": workload,
+                     "avg_launch_ms": round(dom["ms"], 3),
+                     "algorithmic": "FFT-equivalent flops k*(10*H*log2 H + 20*H), H = N/2, "
+                                    "per sequence of the launch",
+                     "mfma_exec_tflops": (round(dom["mfma_exec_flops"] / dom["ms"] / 1e9, 1)
+                                          if dom["mfma_exec_flops"] else None)},
+        "roofline_hbm": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBPS,
+                         "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBPS, 4),
+                         "kernel": dom["kernel"],
+                         "algorithmic": "packed bytes 5/8*(L+1) per sequence of the launch"},
         "q500": r500,
         "q500_linear4": r500l,
         "cpu_baseline": cpu,

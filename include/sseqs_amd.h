@@ -122,6 +122,10 @@ int sseqs_db_prefilter(const uint8_t* packed, const int64_t* seq_off, const int3
                        const float* lutA, const float* lutB, int vocab_size, int k, int N,
                        int maxlen, float* scores, hipStream_t stream);
 
+/* Name of the kernel sseqs_db_prefilter launches for these sizes (for
+ * profiles and reports), written to name[0..name_len).  Host only. */
+int sseqs_db_prefilter_kernel(int query_len, int k, int N, int maxlen, char* name, int name_len);
+
 /* Hit gather: row i of out (n rows of `width` bytes) receives the first
  * min(seq_len[ids[i]], width) residues of database sequence ids[i] as
  * ASCII, and out_len[i] that count.  Database-resident form of the hit

@@ -18,6 +18,7 @@
 // read.  B fragments are table lookups of the staged residue codes.  Inputs
 // are fp16, accumulation fp32, so there is no fp16 overflow (the reference's
 // all-fp16 evaluation saturates to inf on strong hits).
+#include <cstdio>
 #include <type_traits>
 
 #include "common.h"
@@ -565,7 +566,7 @@ __global__ __launch_bounds__(64) void k_corr_db4(const uint8_t* __restrict__ pac
 // stage buffer (DBL) and the index entries of the one after are loaded.
 //
 // Dynamic LDS: btab4 40 x 8 B | qtab 4*(N+8) fp16 during set-up, then
-// (not FULL) save (R-1) x 64 lanes x 16 B | 1 or 2 (DBL) stages of CAPB+64
+// (not FULL) save R x 64 lanes x 16 B | 1 or 2 (DBL) stages of CAPB+64
 // bytes (rounded to 16).
 // ---------------------------------------------------------------------------
 __device__ inline float max4(f32x4 v) { return fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3])); }
@@ -584,8 +585,8 @@ __global__ __launch_bounds__(64, (R <= 8 ? 2 : 1)) void k_corr_band(const uint8_
     uint64_t* btab4 = reinterpret_cast<uint64_t*>(smem);  // [40] x 4 fp16
     uint8_t* region = smem + CORR_NB * 8;
     _Float16* qtab = reinterpret_cast<_Float16*>(region);  // set-up only
-    f32x4* save = reinterpret_cast<f32x4*>(region);        // [R-1][64]
-    uint8_t* stg = region + (FULL ? 0 : (R - 1) * 64 * 16);
+    f32x4* save = reinterpret_cast<f32x4*>(region);        // [R][64]
+    uint8_t* stg = region + (FULL ? 0 : R * 64 * 16);
 
     const int lane = threadIdx.x;
     const int QS = N + 8;
@@ -718,71 +719,87 @@ __global__ __launch_bounds__(64, (R <= 8 ? 2 : 1)) void k_corr_band(const uint8_
         const int Le = cur.Le, C = cur.C;
         const int64_t relbit = 5 * o - 8 * cur.b_lo;
 
-        f32x4 acc[R];
+        // B fragments (4 ranks) of chunk v: this lane's 8 positions 32v+8g..
+        auto decode = [&](auto fast_tag, int v, half8* b) {
+            const int pb = 32 * v + 8 * g;
+            uint32_t sym[8];
+            if constexpr (decltype(fast_tag)::value) {
+                stage_syms(sbuf, relbit + 5 * (int64_t)pb, sym);
+            } else {
 #pragma unroll
-        for (int t = 0; t < R; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-        float best = -INFINITY;
-        const int x = FULL ? 0 : max(0, C - ND + Rn - 1);
-        for (int vb = 0; vb < C; vb += R) {
+                for (int e = 0; e < 8; ++e)
+                    sym[e] = pb + e < Le ? packed_sym(packed, o + pb + e) - 64u : 0u;
+            }
 #pragma unroll
-            for (int s = 0; s < R; ++s) {
-                const int v = vb + s;
-                if (v < C) {
-                    const int pb = 32 * v + 8 * g;
-                    uint32_t sym[8];
-                    if (cur.fast) {
-                        stage_syms(sbuf, relbit + 5 * (int64_t)pb, sym);
-                    } else {
+            for (int e = 0; e < 8; ++e) {
+                const uint32_t ix = pb + e < Le ? sym[e] : (uint32_t)CORR_PAD;
+                union {
+                    uint64_t u;
+                    _Float16 h4[4];
+                } w;
+                w.u = btab4[ix];
 #pragma unroll
-                        for (int e = 0; e < 8; ++e)
-                            sym[e] = pb + e < Le ? packed_sym(packed, o + pb + e) - 64u : 0u;
-                    }
-                    half8 b[K];
+                for (int r = 0; r < K; ++r) b[r][e] = w.h4[r];
+            }
+        };
+        // The tile's chunks; chunk v+1 is decoded in the same basic block as
+        // chunk v's MFMAs, so the two interleave.  Two instances: symbols
+        // from the stage (fast) or from global memory.
+        auto run = [&](auto fast_tag) -> float {
+            f32x4 acc[R];
 #pragma unroll
-                    for (int e = 0; e < 8; ++e) {
-                        const uint32_t ix = pb + e < Le ? sym[e] : (uint32_t)CORR_PAD;
-                        union {
-                            uint64_t u;
-                            _Float16 h4[4];
-                        } w;
-                        w.u = btab4[ix];
+            for (int t = 0; t < R; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+            float best = -INFINITY;
+            const int x = FULL ? 0 : max(0, C - ND + Rn - 1);
+            half8 bc[K];
+            if (C > 0) decode(fast_tag, 0, bc);
+            for (int vb = 0; vb < C; vb += R) {
 #pragma unroll
-                        for (int r = 0; r < K; ++r) b[r][e] = w.h4[r];
-                    }
-                    // consecutive MFMAs go to different accumulators
+                for (int s = 0; s < R; ++s) {
+                    const int v = vb + s;
+                    if (v < C) {
+                        half8 bn[K];
+                        decode(fast_tag, min(v + 1, C - 1), bn);
+                        // consecutive MFMAs go to different accumulators
 #pragma unroll
-                    for (int r = 0; r < K; ++r)
+                        for (int r = 0; r < K; ++r)
 #pragma unroll
-                        for (int j = 0; j < R; ++j)
-                            acc[(s + j) % R] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                                A[j][r], b[r], acc[(s + j) % R], 0, 0, 0);
-                    if (!FULL) {
-                        // relative m-tile v is complete
-                        if (v < x)
-                            save[v * 64 + lane] = acc[s];
-                        else
-                            best = fmaxf(best, max4(acc[s]));
-                        acc[s] = f32x4{0.f, 0.f, 0.f, 0.f};
+                            for (int j = 0; j < R; ++j)
+                                acc[(s + j) % R] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                                    A[j][r], bc[r], acc[(s + j) % R], 0, 0, 0);
+                        if (!FULL) {
+                            // relative m-tile v is complete: park it (slot R-1
+                            // takes the stores of the ones that are not)
+                            const bool park = v < x;
+                            save[(park ? v : R - 1) * 64 + lane] = acc[s];
+                            const float m = max4(acc[s]);
+                            best = park ? best : fmaxf(best, m);
+                            acc[s] = f32x4{0.f, 0.f, 0.f, 0.f};
+                        }
+#pragma unroll
+                        for (int r = 0; r < K; ++r) bc[r] = bn[r];
                     }
                 }
             }
-        }
-        if (FULL) {
+            if (FULL) {
 #pragma unroll
-            for (int t = 0; t < R; ++t) best = fmaxf(best, max4(acc[t]));
-        } else {
-            // live: rel = C .. C+R-2 in slots rel mod R
+                for (int t = 0; t < R; ++t) best = fmaxf(best, max4(acc[t]));
+            } else {
+                // live: rel = C .. C+R-2 in slots rel mod R
 #pragma unroll
-            for (int t = 0; t < R; ++t) {
-                const int rel = C + ((t - C) % R + R) % R;
-                if (rel == C + R - 1) continue;
-                if (rel < ND)
-                    best = fmaxf(best, max4(acc[t]));
-                else if (rel - ND < x)
-                    best = fmaxf(best, max4(acc[t] + save[(rel - ND) * 64 + lane]));
+                for (int t = 0; t < R; ++t) {
+                    const int rel = C + ((t - C) % R + R) % R;
+                    if (rel == C + R - 1) continue;
+                    if (rel < ND)
+                        best = fmaxf(best, max4(acc[t]));
+                    else if (rel - ND < x)
+                        best = fmaxf(best, max4(acc[t] + save[(rel - ND) * 64 + lane]));
+                }
+                if (C + R - 1 < ND) best = fmaxf(best, 0.f);  // m-tiles no chunk reaches
             }
-            if (C + R - 1 < ND) best = fmaxf(best, 0.f);  // m-tiles no chunk reaches
-        }
+            return best;
+        };
+        float best = cur.fast ? run(std::true_type{}) : run(std::false_type{});
         // the four lane groups hold different rows of the same columns
         best = fmaxf(best, __shfl_xor(best, 16, 64));
         best = fmaxf(best, __shfl_xor(best, 32, 64));
@@ -865,6 +882,93 @@ extern "C" int launch_warp_corr(uint16_t* corr, const uint8_t* qrev, const float
                                                    reinterpret_cast<_Float16*>(corr), stream);
 }
 
+// Which kernel and launch shape sseqs_db_prefilter uses for a length range.
+struct PrefilterPlan {
+    enum Kind { BAND, DB4, DB } kind;
+    int R = 0;        // band: ring size (diagonals held in registers)
+    bool full = false, dbl = false;
+    int d0 = 0, rn = 0;
+    int nmt = 1;      // dense kernels: m-tiles per pass
+    int capb = 0;     // stage capacity (bytes)
+    size_t lds = 0;   // dynamic LDS (bytes)
+    int tile = 32;    // sequences per tile
+};
+
+static PrefilterPlan plan_prefilter(int query_len, int k, int N, int mlen) {
+    PrefilterPlan P;
+    const int Lqe = query_len < N ? query_len : N;
+    if (k == 4 && N >= 128 && Lqe > 0 && !getenv("SSEQS_CORR_NO_BAND")) {
+        band_diagonals(N, Lqe, &P.d0, &P.rn);
+        const int ND = N / 32;
+        P.full = P.rn == ND;
+        if (P.full) {
+            P.R = ND <= 16 ? ND : 0;
+        } else {
+            for (int c : {4, 6, 8, 12, 16, 18})
+                if (!P.R && P.rn <= c) P.R = c;
+        }
+        if (P.R) {
+            P.kind = PrefilterPlan::BAND;
+            P.tile = 16;
+            // a tile of 16 length-sorted sequences spans about 16 of them
+            const int span = (min(mlen, N) + 31) & ~31;
+            P.capb = (int)((((int64_t)16 * (span + 1) * 5) / 8 + 48 + 15) & ~15);
+            const size_t qt = (size_t)4 * (N + 8) * 2;
+            const size_t sbytes = (size_t)((P.capb + 64 + 15) & ~15);
+            const size_t save = P.full ? 0 : (size_t)P.R * 64 * 16;
+            auto lds_for = [&](int nbuf) {
+                const size_t run = save + nbuf * sbytes;
+                return ((size_t)CORR_NB * 8 + (qt > run ? qt : run) + 15) & ~(size_t)15;
+            };
+            // two stage buffers while 8 blocks (2 waves per SIMD) still fit a CU
+            P.dbl = lds_for(2) <= 20 * 1024;
+            if (const char* e = getenv("SSEQS_CORR_DBL")) P.dbl = atoi(e) != 0;
+            P.lds = lds_for(P.dbl ? 2 : 1);
+            return P;
+        }
+    }
+    P.kind = k == 4 ? PrefilterPlan::DB4 : PrefilterPlan::DB;
+    const int KPmax = (min(mlen, N) + 15) & ~15;
+    // a length-sorted tile spans about 32 sequences of at most KPmax symbols
+    P.capb = (int)((((int64_t)33 * (KPmax + 17) * 5) / 8 + 64 + 15) & ~15);
+    P.lds = ((size_t)k * (N + 8) * 2 + (size_t)k * CORR_NB * 2 + (size_t)P.capb + 64 + 15) &
+            ~(size_t)15;
+    P.nmt = N >= 64 ? N / 64 : 1;
+    // m-tiles per pass: more reuse of each decoded B fragment against more
+    // registers (8 -> 1 wave per SIMD, 4 -> 2).  Measured on gfx950 with
+    // rank 4: N = 512 runs fastest at 4, N = 1024 and 2048 at 8.
+    if (P.nmt > 4 && N <= 512) P.nmt = 4;
+    if (P.nmt > 8) P.nmt = 8;
+    // SSEQS_CORR_MT caps it further (tuning knob; results do not depend on it)
+    if (const char* e = getenv("SSEQS_CORR_MT")) {
+        int cap = atoi(e);
+        while (P.nmt > 1 && P.nmt > cap) P.nmt >>= 1;
+    }
+    return P;
+}
+
+static std::string plan_name(const PrefilterPlan& P) {
+    char b[64];
+    if (P.kind == PrefilterPlan::BAND)
+        snprintf(b, sizeof b, "k_corr_band<%d, %s, %s>", P.R, P.full ? "true" : "false",
+                 P.dbl ? "true" : "false");
+    else
+        snprintf(b, sizeof b, "%s<%d>", P.kind == PrefilterPlan::DB4 ? "k_corr_db4" : "k_corr_db",
+                 P.nmt);
+    return b;
+}
+
+extern "C" int sseqs_db_prefilter_kernel(int query_len, int k, int N, int maxlen, char* name,
+                                         int name_len) {
+    if (N < 16 || N > 2048 || (N & (N - 1)))
+        return fail(-1, "db_prefilter_kernel: N must be a power of two in 16..2048");
+    if (k < 1 || k > 20) return fail(-1, "db_prefilter_kernel: k must be 1..20");
+    if (!name || name_len < 1) return fail(-1, "db_prefilter_kernel: no output buffer");
+    const std::string s = plan_name(plan_prefilter(query_len, k, N, maxlen < 0 ? 0 : maxlen));
+    snprintf(name, (size_t)name_len, "%s", s.c_str());
+    return 0;
+}
+
 extern "C" int sseqs_db_prefilter(const uint8_t* packed, const int64_t* seq_off,
                                   const int32_t* seq_len, int64_t s_begin, int64_t s_end,
                                   const uint8_t* qrev, int query_len, const float* lutA,
@@ -878,114 +982,66 @@ extern "C" int sseqs_db_prefilter(const uint8_t* packed, const int64_t* seq_off,
     if (k < 1 || k > 20) return fail(-1, "db_prefilter: k must be 1..20");
     if (vocab_size < 1) return fail(-1, "db_prefilter: vocab_size must be >= 1");
     const int mlen = maxlen < 0 ? 0 : maxlen;
-    const int KPmax = (min(mlen, N) + 15) & ~15;
-    // a length-sorted tile spans about 32 sequences of at most KPmax symbols
-    int CAPB = (int)(((int64_t)33 * (KPmax + 17) * 5) / 8 + 64);
-    CAPB = (CAPB + 15) & ~15;
-    size_t lds = (size_t)k * (N + 8) * 2 + (size_t)k * CORR_NB * 2 + (size_t)CAPB + 64;
-    lds = (lds + 15) & ~(size_t)15;
-    if (lds > 160 * 1024) return fail(-1, "db_prefilter: tables exceed LDS (reduce k or N)");
-    const int64_t tiles = ceil_div(nseq, 32);
+    const PrefilterPlan P = plan_prefilter(query_len, k, N, mlen);
+    if (P.lds > 160 * 1024) return fail(-1, "db_prefilter: tables exceed LDS (reduce k or N)");
+    const int64_t tiles = ceil_div(nseq, P.tile);
     const unsigned grid = (unsigned)(tiles < 8192 ? tiles : 8192);
-    int nmt = N >= 64 ? N / 64 : 1;
-    // m-tiles per pass: more reuse of each decoded B fragment against more
-    // registers (8 -> 1 wave per SIMD, 4 -> 2).  Measured on gfx950 with
-    // rank 4: N = 512 runs fastest at 4, N = 1024 and 2048 at 8.
-    if (nmt > 4 && N <= 512) nmt = 4;
-    // SSEQS_CORR_MT caps it further (tuning knob; results do not depend on it)
-    if (const char* e = getenv("SSEQS_CORR_MT")) {
-        int cap = atoi(e);
-        while (nmt > 1 && nmt > cap) nmt >>= 1;
-    }
-    auto go = [&](auto kern) -> int {
-        if (lds > 64 * 1024 &&
-            hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
-            return fail(-3, "db_prefilter: cannot enable large LDS");
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, stream, packed, seq_off, seq_len,
-                           s_begin, nseq, qrev, query_len, lutA, lutB, vocab_size, k, N, mlen,
-                           CAPB, scores);
-        return check_launch("db_prefilter");
+    auto allow_lds = [&](const void* kern) -> bool {
+        return P.lds <= 64 * 1024 ||
+               hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   (int)P.lds) == hipSuccess;
     };
-    const int Lqe = query_len < N ? query_len : N;
-    if (k == 4 && N >= 128 && Lqe > 0 && !getenv("SSEQS_CORR_NO_BAND")) {
-        int d0 = 0, rn = 0;
-        band_diagonals(N, Lqe, &d0, &rn);
-        const int ND = N / 32;
-        const bool full = rn == ND;
-        int R = 0;
-        if (full) {
-            R = ND <= 8 ? ND : 0;
-        } else {
-            for (int c : {4, 6, 8, 10, 12})
-                if (!R && rn <= c) R = c;
-            // R > 8 runs at one wave per SIMD: worth it only for a narrow band
-            if (R > 8 && 2 * rn > ND) R = 0;
-        }
-        if (R) {
-            // a tile of 16 length-sorted sequences spans about 16 of them
-            const int span = (min(mlen, N) + 31) & ~31;
-            int capb = (int)(((int64_t)16 * (span + 1) * 5) / 8 + 48);
-            capb = (capb + 15) & ~15;
-            const size_t qt = (size_t)4 * (N + 8) * 2;
-            const size_t sbytes = (size_t)((capb + 64 + 15) & ~15);
-            const size_t save = full ? 0 : (size_t)(R - 1) * 64 * 16;
-            auto lds_for = [&](int nbuf) {
-                const size_t run = save + nbuf * sbytes;
-                return ((size_t)CORR_NB * 8 + (qt > run ? qt : run) + 15) & ~(size_t)15;
-            };
-            // two stage buffers while 8 blocks (2 waves per SIMD) still fit a CU
-            bool dbl = lds_for(2) <= 20 * 1024;
-            if (const char* e = getenv("SSEQS_CORR_DBL")) dbl = atoi(e) != 0;
-            const size_t blds = lds_for(dbl ? 2 : 1);
-            const int64_t btiles = ceil_div(nseq, 16);
-            const unsigned bgrid = (unsigned)(btiles < 8192 ? btiles : 8192);
-            auto gob = [&](auto kern) -> int {
-                if (blds > 64 * 1024 &&
-                    hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)blds) != hipSuccess)
-                    return fail(-3, "db_prefilter: cannot enable large LDS");
-                hipLaunchKernelGGL(kern, dim3(bgrid), dim3(64), blds, stream, packed, seq_off,
-                                   seq_len, s_begin, nseq, qrev, query_len, lutA, lutB,
-                                   vocab_size, N, mlen, capb, d0, rn, scores);
-                return check_launch("db_prefilter");
-            };
-            auto pick = [&](auto dbl_tag) -> int {
-                constexpr bool D = decltype(dbl_tag)::value;
-                if (full)
-                    return R == 4 ? gob(&k_corr_band<4, true, D>) : gob(&k_corr_band<8, true, D>);
-                switch (R) {
-                case 4: return gob(&k_corr_band<4, false, D>);
-                case 6: return gob(&k_corr_band<6, false, D>);
-                case 8: return gob(&k_corr_band<8, false, D>);
-                case 10: return gob(&k_corr_band<10, false, D>);
-                default: return gob(&k_corr_band<12, false, D>);
-                }
-            };
-            return dbl ? pick(std::true_type{}) : pick(std::false_type{});
-        }
-    }
-    if (k == 4) {
-        auto go4 = [&](auto kern) -> int {
-            if (lds > 64 * 1024 &&
-                hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    (int)lds) != hipSuccess)
+    if (P.kind == PrefilterPlan::BAND) {
+        auto gob = [&](auto kern) -> int {
+            if (!allow_lds(reinterpret_cast<const void*>(kern)))
                 return fail(-3, "db_prefilter: cannot enable large LDS");
-            hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, stream, packed, seq_off, seq_len,
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(64), P.lds, stream, packed, seq_off, seq_len,
                                s_begin, nseq, qrev, query_len, lutA, lutB, vocab_size, N, mlen,
-                               CAPB, scores);
+                               P.capb, P.d0, P.rn, scores);
             return check_launch("db_prefilter");
         };
-        switch (nmt) {
+        auto pick = [&](auto dbl_tag) -> int {
+            constexpr bool D = decltype(dbl_tag)::value;
+            if (P.full)
+                return P.R == 4   ? gob(&k_corr_band<4, true, D>)
+                       : P.R == 8 ? gob(&k_corr_band<8, true, D>)
+                                  : gob(&k_corr_band<16, true, D>);
+            switch (P.R) {
+            case 4: return gob(&k_corr_band<4, false, D>);
+            case 6: return gob(&k_corr_band<6, false, D>);
+            case 8: return gob(&k_corr_band<8, false, D>);
+            case 12: return gob(&k_corr_band<12, false, D>);
+            case 16: return gob(&k_corr_band<16, false, D>);
+            default: return gob(&k_corr_band<18, false, D>);
+            }
+        };
+        return P.dbl ? pick(std::true_type{}) : pick(std::false_type{});
+    }
+    if (P.kind == PrefilterPlan::DB4) {
+        auto go4 = [&](auto kern) -> int {
+            if (!allow_lds(reinterpret_cast<const void*>(kern)))
+                return fail(-3, "db_prefilter: cannot enable large LDS");
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(64), P.lds, stream, packed, seq_off, seq_len,
+                               s_begin, nseq, qrev, query_len, lutA, lutB, vocab_size, N, mlen,
+                               P.capb, scores);
+            return check_launch("db_prefilter");
+        };
+        switch (P.nmt) {
         case 1: return go4(&k_corr_db4<1>);
         case 2: return go4(&k_corr_db4<2>);
         case 4: return go4(&k_corr_db4<4>);
         default: return go4(&k_corr_db4<8>);
         }
     }
-    switch (nmt) {
+    auto go = [&](auto kern) -> int {
+        if (!allow_lds(reinterpret_cast<const void*>(kern)))
+            return fail(-3, "db_prefilter: cannot enable large LDS");
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(64), P.lds, stream, packed, seq_off, seq_len,
+                           s_begin, nseq, qrev, query_len, lutA, lutB, vocab_size, k, N, mlen,
+                           P.capb, scores);
+        return check_launch("db_prefilter");
+    };
+    switch (P.nmt) {
     case 1: return go(&k_corr_db<1>);
     case 2: return go(&k_corr_db<2>);
     case 4: return go(&k_corr_db<4>);

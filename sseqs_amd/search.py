@@ -18,6 +18,7 @@ unspecified in the reference).
 """
 from __future__ import annotations
 
+import ctypes
 import math
 import time
 from dataclasses import dataclass, field
@@ -43,17 +44,32 @@ class GlobalHits:
     timings: dict = field(default_factory=dict)
 
 
-def prefilter_scores(db: SeqDB, q: PreparedQuery, out: torch.Tensor | None = None) -> torch.Tensor:
-    """Prefilter score of every database sequence (float32, index order)."""
+def prefilter_scores(db: SeqDB, q: PreparedQuery, out: torch.Tensor | None = None,
+                     events: list | None = None) -> torch.Tensor:
+    """Prefilter score of every database sequence (float32, index order).
+    ``events``: optional list of len(length ranges) + 1 torch events, recorded
+    on the launch stream before the first and after every range's launch."""
     n = db.n
     if out is None:
         out = torch.empty(n, dtype=torch.float32, device=db.device)
     st = stream_of(db.device)
-    for N, b, e in db.length_ranges(q.length):
+    if events is not None:
+        events[0].record()
+    for i, (N, b, e) in enumerate(db.length_ranges(q.length)):
         call("sseqs_db_prefilter", ptr(db.packed), ptr(db.seq_off), ptr(db.seq_len), b, e,
              ptr(q.qrev), q.length, ptr(q.lut_a), ptr(q.lut_b_flat), 21, q.k, N, N,
              out.data_ptr() + 4 * b, st)
+        if events is not None:
+            events[i + 1].record()
     return out
+
+
+def prefilter_kernel_name(q_len: int, k: int, N: int, maxlen: int | None = None) -> str:
+    """Kernel that sseqs_db_prefilter launches for one length range."""
+    buf = ctypes.create_string_buffer(64)
+    call("sseqs_db_prefilter_kernel", int(q_len), int(k), int(N),
+         int(N if maxlen is None else maxlen), buf, 64)
+    return buf.value.decode()
 
 
 def sw_affine_ids(db: SeqDB, q: PreparedQuery, ids: torch.Tensor, gap_open: int = 11,
@@ -83,7 +99,7 @@ def sw_linear_ids(db: SeqDB, q: PreparedQuery, ids: torch.Tensor, gap: int = -4)
 def global_search(db: SeqDB, q: PreparedQuery, top_prefilter: int = 100_000,
                   top_hits: int = 4096, gap_open: int = 11, gap_extend: int = 1,
                   scores_buf: torch.Tensor | None = None,
-                  linear_gap: int | None = None) -> GlobalHits:
+                  linear_gap: int | None = None, events: list | None = None) -> GlobalHits:
     """The whole-database cascade.  With ``linear_gap`` set, the candidates
     are scored with linear-gap SW (that penalty per gap position) instead of
     affine SW."""
@@ -91,7 +107,7 @@ def global_search(db: SeqDB, q: PreparedQuery, top_prefilter: int = 100_000,
         raise ValueError("query has no residues")
     if q.codes.numel() > 1024:
         raise ValueError("queries longer than 1024 residues are not supported")
-    pre = prefilter_scores(db, q, scores_buf)
+    pre = prefilter_scores(db, q, scores_buf, events)
     kc = min(int(top_prefilter), db.n)
     cand = ops.topk_indices(pre, kc, sorted=False)
     if linear_gap is None:

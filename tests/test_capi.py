@@ -47,3 +47,26 @@ def test_argument_errors_need_no_gpu():
     assert lib.launch_warp_corr(None, None, None, None, None, None, None, None, 4, 21, 4, 10, 10,
                                 100, None, None) == -1
     assert b"power of two" in lib.sseqs_last_error()
+
+
+def test_prefilter_kernel_plan_needs_no_gpu():
+    """The prefilter's kernel choice per length range (host-side plan)."""
+    from sseqs_amd import _lib
+    if not _lib.LIB_PATH.exists():
+        pytest.skip("library not built")
+    lib = _lib.load()
+    buf = ctypes.create_string_buffer(64)
+
+    def name(lq, k, n):
+        assert lib.sseqs_db_prefilter_kernel(lq, k, n, n, buf, 64) == 0
+        return buf.value.decode()
+
+    # q_len 200: every one of the 8 diagonals is nonzero at N = 256, 8 of 16 at 512
+    assert name(200, 4, 256) == "k_corr_band<8, true, true>"
+    assert name(200, 4, 512) == "k_corr_band<8, false, true>"
+    assert name(200, 4, 1024) == "k_corr_band<8, false, false>"
+    assert name(500, 4, 512) == "k_corr_band<16, true, true>"
+    assert name(500, 4, 1024).startswith("k_corr_band<18, false")
+    assert name(200, 3, 256) == "k_corr_db<4>"
+    assert name(50, 4, 64) == "k_corr_db4<1>"
+    assert lib.sseqs_db_prefilter_kernel(200, 4, 300, 300, buf, 64) == -1

@@ -193,4 +193,5 @@ def test_server_ticket_roundtrip(orc, world, tmp_path, monkeypatch):
     rows = direct.read_text().replace("@", "X").split("\n")
     rows[0] = ">101"
     assert got == "\n".join(rows)
-    assert got.count(">") > 10
+    # one batch (bs = 1e8 bytes) keeps top_sw_affine = 2 hits
+    assert got.count(">") == 3
