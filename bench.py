@@ -55,9 +55,9 @@ def cpu_baseline(db, query: str, target_s: float = 15.0) -> dict:
         rows, ln = db.gather(torch.from_numpy(ids).to(db.device), 1024)
         # sequences back to back, so offsets stay far below 2^31 (the
         # oracle's prefilter takes int32 offsets, like launch_warp_corr)
-        keep = torch.arange(rows.shape[1], device=rows.device)[None, :] < ln[:, None]
-        flat = rows[keep].cpu().numpy()
-        ln = ln.cpu().numpy()
+        rows, ln = rows.cpu().numpy(), ln.cpu().numpy()
+        flat = rows[np.arange(rows.shape[1])[None, :] < ln[:, None]]
+        del rows
         off = np.concatenate([[0], np.cumsum(ln.astype(np.int64))[:-1]])
         if flat.size >= 2**31:
             raise ValueError("CPU baseline sample too large for int32 offsets")
@@ -78,7 +78,7 @@ def cpu_baseline(db, query: str, target_s: float = 15.0) -> dict:
     nsamp = 2000
     dt, res = run(nsamp)
     for _ in range(2):  # small samples overstate the per-sequence cost
-        nsamp = int(min(db.n, 4_000_000, max(2000, nsamp * target_s / max(dt, 1e-3))))
+        nsamp = int(min(db.n, 2_000_000, max(2000, nsamp * target_s / max(dt, 1e-3))))
         dt, res = run(nsamp)
         if dt > 0.5 * target_s:
             break
@@ -90,14 +90,15 @@ def cpu_baseline(db, query: str, target_s: float = 15.0) -> dict:
 
 
 def copy_probe(nbytes: int, reps: int) -> None:
-    """A plain device copy of known size, for calibrating rocprofv3's
-    FETCH_SIZE / WRITE_SIZE counters against true bytes moved."""
+    """A plain elementwise pass of known size (reads and writes nbytes), for
+    calibrating rocprofv3's FETCH_SIZE / WRITE_SIZE counters against the true
+    bytes moved."""
     import torch
     dev = torch.device("cuda:0")
     src = torch.ones(nbytes, dtype=torch.uint8, device=dev)
     dst = torch.empty_like(src)
     for _ in range(reps):
-        dst.copy_(src)
+        torch.add(src, 1, out=dst)  # one elementwise kernel: reads and writes nbytes
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
 
