@@ -543,12 +543,15 @@ __global__ __launch_bounds__(64) void k_corr_db4(const uint8_t* __restrict__ pac
 // Rank-4 band kernel on v_mfma_f32_16x16x32_f16.
 //
 // Rows are 16 even shifts (m-tile I: m = 2i, i in [16I, 16I+16)), columns 16
-// sequences, K = 32 positions (chunk c: p in [32c, 32c+32)).  The A block of
-// (I, c) depends only on the diagonal d = (I - c) mod ND, ND = N/32, and is
-// nonzero only for a circular run of Rn diagonals starting at d0, because
-// the query occupies Lq of the N correlation lags.  The A fragments of those
-// diagonals (all 4 ranks) stay in registers for the whole kernel, so each
-// chunk is decoded once and feeds Rn*4 MFMAs and nothing else.
+// sequences.  A chunk c covers positions [32c, 32c+32) and takes 4 MFMAs per
+// m-tile, each with K = 8 positions x 4 ranks: ranks sit next to each other
+// in K, so a B fragment is two 8-byte table entries (4 ranks of 2 symbols)
+// and an A fragment is 16 contiguous bytes of the rank-interleaved query
+// table.  The A block of (I, c) depends only on the diagonal
+// d = (I - c) mod ND, ND = N/32, and is nonzero only for a circular run of
+// Rn diagonals starting at d0, because the query occupies Lq of the N
+// correlation lags.  The A fragments of those diagonals stay in registers
+// for the whole kernel, so each chunk is decoded once and feeds Rn*4 MFMAs.
 //
 // Accumulators form a ring over the R m-tiles a chunk can reach.  With
 // relative m-tile index rel = I - d0 (before reduction mod ND), chunk v
@@ -565,7 +568,7 @@ __global__ __launch_bounds__(64) void k_corr_db4(const uint8_t* __restrict__ pac
 // bytes are copied global -> LDS (global_load_lds_dwordx4) into the second
 // stage buffer (DBL) and the index entries of the one after are loaded.
 //
-// Dynamic LDS: btab4 40 x 8 B | qtab 4*(N+8) fp16 during set-up, then
+// Dynamic LDS: btab4 40 x 8 B | qtab4 (N+8) x 8 B during set-up, then
 // (not FULL) save R x 64 lanes x 16 B | 1 or 2 (DBL) stages of CAPB+64
 // bytes (rounded to 16).
 // ---------------------------------------------------------------------------
@@ -584,18 +587,26 @@ __global__ __launch_bounds__(64, (R <= 8 ? 2 : 1)) void k_corr_band(const uint8_
     extern __shared__ __align__(16) uint8_t smem[];
     uint64_t* btab4 = reinterpret_cast<uint64_t*>(smem);  // [40] x 4 fp16
     uint8_t* region = smem + CORR_NB * 8;
-    _Float16* qtab = reinterpret_cast<_Float16*>(region);  // set-up only
-    f32x4* save = reinterpret_cast<f32x4*>(region);        // [R][64]
+    uint64_t* qtab4 = reinterpret_cast<uint64_t*>(region);  // [N+8] x 4 fp16, set-up only
+    f32x4* save = reinterpret_cast<f32x4*>(region);         // [R][64]
     uint8_t* stg = region + (FULL ? 0 : R * 64 * 16);
 
     const int lane = threadIdx.x;
-    const int QS = N + 8;
     const int Lqe = min(Lq, N);
-    for (int idx = lane; idx < K * QS; idx += 64) {
-        int r = idx / QS, u = idx - r * QS;
-        int t = (N - (u & (N - 1))) & (N - 1);
-        qtab[idx] = (_Float16)(t < Lqe ? 0.5f * lutA[r * V + (int)qrev[t]] : 0.f);
+    // query table, ranks interleaved: qtab4[u] = 0.5 * A[r][qrev[(-u) mod N]], r = 0..3
+    for (int u = lane; u < N + 8; u += 64) {
+        const int t = (N - (u & (N - 1))) & (N - 1);
+        union {
+            _Float16 h[4];
+            uint64_t w;
+        } e;
+#pragma unroll
+        for (int r = 0; r < K; ++r)
+            e.h[r] = (_Float16)(t < Lqe ? 0.5f * lutA[r * V + (int)qrev[t]] : 0.f);
+        qtab4[u] = e.w;
     }
+    // database table by 5-bit symbol.  Symbol 0 ('@') never occurs inside a
+    // sequence, so its entry is zero and stands for masked positions.
     for (int c = lane; c < CORR_NB; c += 64) {
         union {
             _Float16 h[4];
@@ -603,31 +614,31 @@ __global__ __launch_bounds__(64, (R <= 8 ? 2 : 1)) void k_corr_band(const uint8_
         } e;
 #pragma unroll
         for (int r = 0; r < K; ++r)
-            e.h[r] = (_Float16)(c < 32 ? lutB[r * V + corr_code((uint32_t)c + 64u)] : 0.f);
+            e.h[r] = (_Float16)(c > 0 && c < 32 ? lutB[r * V + corr_code((uint32_t)c + 64u)]
+                                                : 0.f);
         btab4[c] = e.u;
     }
     __syncthreads();
 
     // lane (col, g): A row col, B/C column col, positions 8g..8g+7 of a chunk
     const int col = lane & 15, g = lane >> 4;
-    // A fragment of diagonal d0+j, rank r: T[16I+col][32c+8g+x] =
-    // qtab[(8g + x - 2*col - 32*(I-c)) mod N], x = 0..7
-    half8 A[R][K];
+    // K is (position, rank) interleaved: element x of lane (col, g) in
+    // sub-chunk q of chunk c is position 32c + 8g + 2q + (x >> 2), rank x & 3.
+    // A fragment of diagonal d0+j, sub-chunk q: lags u, u+1 with
+    // u = 8g + 2q - 2*col - 32*(I-c) (mod N), i.e. 16 bytes of qtab4.
+    half8 A[R][4];
 #pragma unroll
     for (int j = 0; j < R; ++j) {
-        const int u = (8 * g - 2 * col - 32 * (d0 + j)) & (N - 1);
 #pragma unroll
-        for (int r = 0; r < K; ++r) {
-            const uint32_t* q32 = reinterpret_cast<const uint32_t*>(qtab + r * QS + u);
+        for (int q = 0; q < 4; ++q) {
+            const int u = (8 * g + 2 * q - 2 * col - 32 * (d0 + j)) & (N - 1);
             union {
-                uint32_t w[4];
+                uint64_t w[2];
                 half8 v;
             } a;
-            a.w[0] = q32[0];
-            a.w[1] = q32[1];
-            a.w[2] = q32[2];
-            a.w[3] = q32[3];
-            A[j][r] = a.v;
+            a.w[0] = qtab4[u];
+            a.w[1] = qtab4[u + 1];
+            A[j][q] = a.v;
         }
     }
     __syncthreads();  // the region now holds save + stage
@@ -719,27 +730,38 @@ __global__ __launch_bounds__(64, (R <= 8 ? 2 : 1)) void k_corr_band(const uint8_
         const int Le = cur.Le, C = cur.C;
         const int64_t relbit = 5 * o - 8 * cur.b_lo;
 
-        // B fragments (4 ranks) of chunk v: this lane's 8 positions 32v+8g..
+        // B fragments (4 sub-chunks) of chunk v: this lane's 8 positions
+        // 32v+8g.., symbol 0 past the sequence's end
         auto decode = [&](auto fast_tag, int v, half8* b) {
             const int pb = 32 * v + 8 * g;
             uint32_t sym[8];
             if constexpr (decltype(fast_tag)::value) {
-                stage_syms(sbuf, relbit + 5 * (int64_t)pb, sym);
+                const int64_t bit = relbit + 5 * (int64_t)pb;
+                const int64_t byte = bit >> 3;
+                const uint64_t* s64 = reinterpret_cast<const uint64_t*>(sbuf + (byte & ~7));
+                const uint64_t B0 = __builtin_bswap64(s64[0]);
+                const uint64_t B1 = __builtin_bswap64(s64[1]);
+                const int sh = (int)(byte & 7) * 8 + (int)(bit & 7);
+                uint64_t top = (B0 << sh) | ((B1 >> 1) >> (63 - sh));
+                // keep the first min(max(Le - pb, 0), 8) symbols of the window
+                const int nv = min(max(Le - pb, 0), 8);
+                top &= ~(~0ull >> (5 * nv));
+#pragma unroll
+                for (int e = 0; e < 8; ++e) sym[e] = (uint32_t)(top >> (59 - 5 * e)) & 31u;
             } else {
 #pragma unroll
                 for (int e = 0; e < 8; ++e)
                     sym[e] = pb + e < Le ? packed_sym(packed, o + pb + e) - 64u : 0u;
             }
 #pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const uint32_t ix = pb + e < Le ? sym[e] : (uint32_t)CORR_PAD;
+            for (int q = 0; q < 4; ++q) {
                 union {
-                    uint64_t u;
-                    _Float16 h4[4];
-                } w;
-                w.u = btab4[ix];
-#pragma unroll
-                for (int r = 0; r < K; ++r) b[r][e] = w.h4[r];
+                    uint64_t w[2];
+                    half8 v;
+                } u;
+                u.w[0] = btab4[sym[2 * q]];
+                u.w[1] = btab4[sym[2 * q + 1]];
+                b[q] = u.v;
             }
         };
         // The tile's chunks; chunk v+1 is decoded in the same basic block as
@@ -751,22 +773,25 @@ __global__ __launch_bounds__(64, (R <= 8 ? 2 : 1)) void k_corr_band(const uint8_
             for (int t = 0; t < R; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
             float best = -INFINITY;
             const int x = FULL ? 0 : max(0, C - ND + Rn - 1);
-            half8 bc[K];
-            if (C > 0) decode(fast_tag, 0, bc);
+            // B fragments of the current and the next chunk alternate between
+            // bb[0] and bb[1] (R is even, so chunk v always uses bb[v & 1])
+            static_assert(R % 2 == 0, "ring size must be even");
+            half8 bb[2][4];
+            if (C > 0) decode(fast_tag, 0, bb[0]);
             for (int vb = 0; vb < C; vb += R) {
 #pragma unroll
                 for (int s = 0; s < R; ++s) {
                     const int v = vb + s;
                     if (v < C) {
-                        half8 bn[K];
-                        decode(fast_tag, min(v + 1, C - 1), bn);
+                        half8* bc = bb[s & 1];
+                        decode(fast_tag, min(v + 1, C - 1), bb[(s + 1) & 1]);
                         // consecutive MFMAs go to different accumulators
 #pragma unroll
-                        for (int r = 0; r < K; ++r)
+                        for (int q = 0; q < 4; ++q)
 #pragma unroll
                             for (int j = 0; j < R; ++j)
                                 acc[(s + j) % R] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                                    A[j][r], bc[r], acc[(s + j) % R], 0, 0, 0);
+                                    A[j][q], bc[q], acc[(s + j) % R], 0, 0, 0);
                         if (!FULL) {
                             // relative m-tile v is complete: park it (slot R-1
                             // takes the stores of the ones that are not)
@@ -776,8 +801,6 @@ __global__ __launch_bounds__(64, (R <= 8 ? 2 : 1)) void k_corr_band(const uint8_
                             best = park ? best : fmaxf(best, m);
                             acc[s] = f32x4{0.f, 0.f, 0.f, 0.f};
                         }
-#pragma unroll
-                        for (int r = 0; r < K; ++r) bc[r] = bn[r];
                     }
                 }
             }
