@@ -112,7 +112,7 @@ def load_traffic(kernel: str, workload: str):
             rec = json.load(fh).get(f"{kernel} | {workload}")
     except (OSError, ValueError):
         return None
-    return rec
+    return int(rec["bytes"]) if rec else None
 
 
 def main() -> int:
@@ -307,6 +307,7 @@ def main() -> int:
                      "peak": MFMA_F16_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(ach / MFMA_F16_PEAK_TFLOPS, 4),
                      "traffic": traffic,
+                     "traffic_source": "profiles/prefilter_pmc_200M.md" if traffic else None,
                      "kernel": dom["kernel"], "launch": workload,
                      "avg_launch_ms": round(dom["ms"], 3),
                      "algorithmic": "FFT-equivalent flops k*(10*H*log2 H + 20*H), H = N/2, "
