@@ -72,7 +72,8 @@ int launch_warp_corr(uint16_t* corr, const uint8_t* query_reversed_indices,
 /* ---------------------------------------------------------- Smith-Waterman -- */
 
 /* Linear-gap local alignment score of M targets against the query
- * (codes 0..19, length 1..1024).  Target g is
+ * (codes 0..19, length 1..2048; the reference stopped at 1024,
+ * sseqs/sw.cu:355-369).  Target g is
  * ascii[P[good_idx[g]-1]+1 .. P[good_idx[g]]) with P = `lengths` (the
  * reference's name for the delimiter-position array); P[-1] is taken as -1.
  * out[g] = -1 for an empty query or target, like the reference.
@@ -135,7 +136,7 @@ int sseqs_db_gather(const uint8_t* packed, const int64_t* seq_off, const int32_t
                     hipStream_t stream);
 
 /* Affine score (reference stripe rule) of database sequences cand[0..n) of
- * the resident database against the query (codes 0..19, length 1..1024).
+ * the resident database against the query (codes 0..19, length 1..2048).
  * out[i] = -1 for an empty sequence. */
 int sseqs_db_sw_affine(const uint8_t* packed, const int64_t* seq_off, const int32_t* seq_len,
                        const int64_t* cand, int64_t n, const uint8_t* query, int query_len,

@@ -178,10 +178,14 @@ static int launch_sw(const Src& src, const uint8_t* q, int Lq, int64_t n, int p1
         SW_CASE(9) SW_CASE(10) SW_CASE(11) SW_CASE(12) SW_CASE(13) SW_CASE(14) SW_CASE(15)
         SW_CASE(16) SW_CASE(17) SW_CASE(18) SW_CASE(19) SW_CASE(20) SW_CASE(21) SW_CASE(22)
         SW_CASE(23) SW_CASE(24) SW_CASE(25) SW_CASE(26) SW_CASE(27) SW_CASE(28) SW_CASE(29)
-        SW_CASE(30) SW_CASE(31) SW_CASE(32)
+        SW_CASE(30) SW_CASE(31) SW_CASE(32) SW_CASE(33) SW_CASE(34) SW_CASE(35) SW_CASE(36)
+        SW_CASE(37) SW_CASE(38) SW_CASE(39) SW_CASE(40) SW_CASE(41) SW_CASE(42) SW_CASE(43)
+        SW_CASE(44) SW_CASE(45) SW_CASE(46) SW_CASE(47) SW_CASE(48) SW_CASE(49) SW_CASE(50)
+        SW_CASE(51) SW_CASE(52) SW_CASE(53) SW_CASE(54) SW_CASE(55) SW_CASE(56) SW_CASE(57)
+        SW_CASE(58) SW_CASE(59) SW_CASE(60) SW_CASE(61) SW_CASE(62) SW_CASE(63) SW_CASE(64)
 #undef SW_CASE
     default:
-        return fail(-1, "sw: query length must be 1..1024");
+        return fail(-1, "sw: query length must be 1..2048");
     }
     return check_launch(AFFINE ? "sw affine" : "sw linear");
 }
@@ -206,7 +210,7 @@ extern "C" int launch_sw_cuda_blosum62(const uint8_t* q, int Lq, const int32_t* 
                                        int gap_val, hipStream_t stream) {
     if (n < 0) return fail(-1, "sw_cuda_blosum62: negative count");
     if (n == 0) return 0;
-    if (Lq < 0 || Lq > 1024) return fail(-1, "sw_cuda_blosum62: query length must be 0..1024");
+    if (Lq < 0 || Lq > 2048) return fail(-1, "sw_cuda_blosum62: query length must be 0..2048");
     if (gap_val > 0) return fail(-1, "sw_cuda_blosum62: gap_penalty must be <= 0");
     if (Lq == 0) return fill_minus_one(out, n, stream);
     SrcAscii src{ascii, P, good_idx};
@@ -218,7 +222,7 @@ extern "C" int launch_sw_cuda_affine(const uint8_t* q, int Lq, const int32_t* go
                                      int gap_open, int gap_extend, hipStream_t stream) {
     if (n < 0) return fail(-1, "sw_cuda_affine: negative count");
     if (n == 0) return 0;
-    if (Lq < 0 || Lq > 1024) return fail(-1, "sw_cuda_affine: query length must be 0..1024");
+    if (Lq < 0 || Lq > 2048) return fail(-1, "sw_cuda_affine: query length must be 0..2048");
     if (gap_open < 0 || gap_extend < 0)
         return fail(-1, "sw_cuda_affine: gap_open and gap_extend must be >= 0");
     if (Lq == 0) return fill_minus_one(out, n, stream);
@@ -232,7 +236,7 @@ extern "C" int sseqs_db_sw_affine(const uint8_t* packed, const int64_t* seq_off,
                                   int32_t* out, hipStream_t stream) {
     if (n < 0) return fail(-1, "db_sw_affine: negative count");
     if (n == 0) return 0;
-    if (Lq < 1 || Lq > 1024) return fail(-1, "db_sw_affine: query length must be 1..1024");
+    if (Lq < 1 || Lq > 2048) return fail(-1, "db_sw_affine: query length must be 1..2048");
     if (gap_open < 0 || gap_extend < 0)
         return fail(-1, "db_sw_affine: gap_open and gap_extend must be >= 0");
     SrcPacked src{packed, seq_off, seq_len, cand};
@@ -245,7 +249,7 @@ extern "C" int sseqs_db_sw_linear(const uint8_t* packed, const int64_t* seq_off,
                                   hipStream_t stream) {
     if (n < 0) return fail(-1, "db_sw_linear: negative count");
     if (n == 0) return 0;
-    if (Lq < 1 || Lq > 1024) return fail(-1, "db_sw_linear: query length must be 1..1024");
+    if (Lq < 1 || Lq > 2048) return fail(-1, "db_sw_linear: query length must be 1..2048");
     if (gap_val > 0) return fail(-1, "db_sw_linear: gap_val must be <= 0");
     SrcPacked src{packed, seq_off, seq_len, cand};
     return launch_sw<false>(src, query, Lq, n, gap_val, 0, out, stream);

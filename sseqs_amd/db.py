@@ -24,6 +24,31 @@ from ._lib import call, ptr, stream_of
 # Tail bytes after the stream: symbol reads touch 2 bytes and the prefilter
 # copies 16-byte chunks up to 40 bytes past the last symbol's byte.
 PAD = 64
+# The reference reads its database in chunks of 1 GB (sseqs/msa.py:50-55).
+CHUNK_BYTES = 1_000_000_000
+
+
+@dataclass
+class XbitLayout:
+    starts: list[int]   # byte offset of each file's data in the stream
+    sizes: list[int]    # bytes read from each file
+    total: int          # stream bytes (a multiple of 5: whole 8-symbol groups)
+
+
+def xbit_layout(sizes: list[int], chunks: list[int | None], chunk_bytes: int = CHUNK_BYTES
+                ) -> XbitLayout:
+    """Placement of several .xbit files in one packed stream.  File i
+    contributes its first chunks[i] * chunk_bytes bytes (all of it for None)
+    and is padded with zero bits to a multiple of 5 bytes, so the next file
+    starts on a symbol boundary; the zero symbols decode as '@' (empty runs,
+    which the index skips)."""
+    starts, used, pos = [], [], 0
+    for size, c in zip(sizes, chunks):
+        m = size if c is None else min(size, int(c) * int(chunk_bytes))
+        starts.append(pos)
+        used.append(m)
+        pos += (m + 4) // 5 * 5
+    return XbitLayout(starts, used, pos)
 
 
 @dataclass
@@ -135,26 +160,33 @@ class SeqDB:
         return SeqDB(packed, int(n_syms), off, ln, orig)
 
     @staticmethod
-    def from_xbit(path: str | os.PathLike, chunks: int | None = None, device="cuda") -> "SeqDB":
-        """Load an .xbit file (or its first ``chunks`` x 1e9 bytes, like the
-        reference's CHUNKS, sseqs/msa.py:14,50-55) into HBM and index it."""
+    def from_xbit(path: str | os.PathLike, chunks: int | None = None, device="cuda",
+                  ab_path: str | os.PathLike | None = None, ab_chunks: int | None = 23,
+                  chunk_bytes: int = CHUNK_BYTES) -> "SeqDB":
+        """Load an .xbit file (its first ``chunks`` chunks of ``chunk_bytes``,
+        like the reference's CHUNKS, sseqs/msa.py:14,50-55) into HBM and
+        index it.  ``ab_path``: a second database appended after it (the
+        antibody set of sseqs/msa.py:39-40,57-62, first ``ab_chunks`` chunks).
+        A file shorter than its chunk count is read to its end (the
+        reference left the rest of its pinned chunks uninitialised)."""
         dev = torch.device(device)
-        size = os.path.getsize(path)
-        if chunks is not None:
-            size = min(size, int(chunks) * 1_000_000_000)
-        packed = torch.zeros(size + PAD, dtype=torch.uint8, device=dev)
+        files = [(path, chunks)] + ([(ab_path, ab_chunks)] if ab_path is not None else [])
+        sizes = [os.path.getsize(f) for f, _ in files]
+        lay = xbit_layout(sizes, [c for _, c in files], chunk_bytes)
+        packed = torch.zeros(lay.total + PAD, dtype=torch.uint8, device=dev)
         step = 1 << 28
         host = torch.empty(step, dtype=torch.uint8, pin_memory=True)
-        with open(path, "rb") as fh:
-            pos = 0
-            while pos < size:
-                m = min(step, size - pos)
-                got = fh.readinto(host[:m].numpy())
-                if got != m:
-                    raise IOError(f"short read from {path}")
-                packed[pos: pos + m].copy_(host[:m], non_blocking=False)
-                pos += m
-        return SeqDB.from_packed(packed, size * 8 // 5)
+        for (f, _), start, size in zip(files, lay.starts, lay.sizes):
+            with open(f, "rb") as fh:
+                pos = 0
+                while pos < size:
+                    m = min(step, size - pos)
+                    got = fh.readinto(host[:m].numpy())
+                    if got != m:
+                        raise IOError(f"short read from {f}")
+                    packed[start + pos: start + pos + m].copy_(host[:m], non_blocking=False)
+                    pos += m
+        return SeqDB.from_packed(packed, lay.total * 8 // 5)
 
     @staticmethod
     def synthetic(n: int, queries: list[str], seed: int = 0, device="cuda",
@@ -174,7 +206,8 @@ _default_db: SeqDB | None = None
 
 def default_db(device="cuda") -> SeqDB:
     """Database named by DBPATH (default uniref_bfd_mgy_cf.xbit), first CHUNKS
-    GB (sseqs/msa.py:14,18), loaded on first use."""
+    GB (sseqs/msa.py:14,18), followed by the antibody database named by AB
+    when set, loaded on first use."""
     global _default_db
     if _default_db is None:
         path = os.environ.get("DBPATH", "uniref_bfd_mgy_cf.xbit")
@@ -182,7 +215,9 @@ def default_db(device="cuda") -> SeqDB:
             raise FileNotFoundError(
                 f"Didn't find MSA database at `{path}` (set DBPATH, or pass db=...)")
         chunks = os.environ.get("CHUNKS")
-        _default_db = SeqDB.from_xbit(path, int(chunks) if chunks else None, device=device)
+        ab = os.environ.get("AB") or None  # sseqs/msa.py:39-40,57-62
+        _default_db = SeqDB.from_xbit(path, int(chunks) if chunks else None, device=device,
+                                      ab_path=ab)
     return _default_db
 
 

@@ -6,7 +6,9 @@ at import (``db=`` selects another one); ``cascade="global"`` runs the
 whole-database cascade (global top ``top_prefilter`` by prefilter, affine SW,
 top ``num_msas``) instead of the reference's per-batch one; a batch whose
 cascade would leave fewer than ``top_sw_affine`` affine candidates is
-skipped (the reference raises inside torch.topk there).
+skipped (the reference raises inside torch.topk there); ``hit_width``
+(default 500, as the reference) is the number of residues of each hit that
+is aligned into the a3m, None for whole hits.
 """
 from __future__ import annotations
 
@@ -29,7 +31,8 @@ _last: dict[int, threading.Thread] = {}
 def msa(queries, filenames=None, num_msas=4096, save_raw="", verbose=True, bs=50_000_000,
         fft_rank=4, top_fft=200, top_sw=10, top_sw_affine=2, return_timings=False,
         sync_time=False, dlength=1e6, gpu: int = 0, blocking: bool = True, *,
-        db: SeqDB | None = None, cascade: str = "batch", top_prefilter: int = 100_000):
+        db: SeqDB | None = None, cascade: str = "batch", top_prefilter: int = 100_000,
+        hit_width: int | None = 500):
     if not blocking:  # sseqs/msa.py:88-101
         t = _last.get(gpu)
         if t is not None and t.is_alive():
@@ -73,6 +76,7 @@ def msa(queries, filenames=None, num_msas=4096, save_raw="", verbose=True, bs=50
                         _bar.update(nb / 1e9)
                         _bar.set_description(f"db_len={int(maxl)} q_len={len(_q)}")
                 hits = batch_search(db, query, q, bs=bs, top_fft=top_fft, top_sw=top_sw,
+                                    hit_width=hit_width,
                                     top_sw_affine=top_sw_affine, dlength=dlength,
                                     timings=timings, progress=progress)
                 if bar is not None:
@@ -83,7 +87,9 @@ def msa(queries, filenames=None, num_msas=4096, save_raw="", verbose=True, bs=50
                        [h.fft for h in hits], [h.norm for h in hits]]
             elif cascade == "global":
                 gh = global_search(db, q, top_prefilter=top_prefilter, top_hits=num_msas)
-                seqs = db.strings(gh.ids, 500)
+                width = hit_width if hit_width is not None else max(
+                    1, int(db.seq_len[gh.ids].max().item()) if gh.ids.numel() else 1)
+                seqs = db.strings(gh.ids, width)
                 raw = [query, seqs, gh.scores.cpu().tolist(), [], gh.scores.cpu().tolist()]
             else:
                 raise ValueError(f"unknown cascade {cascade!r}")

@@ -105,8 +105,8 @@ def global_search(db: SeqDB, q: PreparedQuery, top_prefilter: int = 100_000,
     affine SW."""
     if q.codes.numel() == 0:
         raise ValueError("query has no residues")
-    if q.codes.numel() > 1024:
-        raise ValueError("queries longer than 1024 residues are not supported")
+    if q.codes.numel() > 2048:
+        raise ValueError("queries longer than 2048 residues are not supported")
     pre = prefilter_scores(db, q, scores_buf, events)
     kc = min(int(top_prefilter), db.n)
     cand = ops.topk_indices(pre, kc, sorted=False)
@@ -213,8 +213,11 @@ def _nextpow2(x: int) -> int:
 def batch_search(db: SeqDB, query: str, q: PreparedQuery, bs: int = 50_000_000,
                  top_fft: int = 200, top_sw: int = 10, top_sw_affine: int = 2,
                  dlength: float = 1e6, chunk_bytes: int = 1_000_000_000,
-                 timings: dict | None = None, progress=None) -> list[BatchHit]:
-    """The reference cascade of sseqs/msa.py:196-370 on the resident stream."""
+                 timings: dict | None = None, progress=None,
+                 hit_width: int | None = 500) -> list[BatchHit]:
+    """The reference cascade of sseqs/msa.py:196-370 on the resident stream.
+    Hits keep their first ``hit_width`` residues (500 in the reference,
+    sseqs/msa.py:183,367; None keeps them whole)."""
     dev = db.device
     hits: list[BatchHit] = []
     Lq = len(query)
@@ -279,7 +282,9 @@ def batch_search(db: SeqDB, query: str, q: PreparedQuery, bs: int = 50_000_000,
         norm = gap_sel.float() / torch.tensor(float(Lq * maxlen), device=dev).sqrt()
         seg = good_idx[pos]
         hs, hl = fft_off[:-1][seg], lengths[seg]
-        width = min(500, int(hl.max().item()))
+        width = int(hl.max().item())
+        if hit_width is not None:
+            width = min(int(hit_width), width)
         idx = hs[:, None] + torch.arange(width, device=dev)[None, :]
         rows = ascii[idx.clamp_max(ascii.numel() - 1)]
         t["t_sw"] = t.get("t_sw", 0.0) + time.time() - t0

@@ -12,6 +12,8 @@ Routes, as in server.py:30-110,133-169:
   (``pair.a3m`` on the pair route) plus an empty
   ``bfd.mgnify30.metaeuk30.smag30.a3m`` into a tar.gz. The route answers
   ``{"status": "COMPLETE", "id": <tar id>}``.
+* ``GET /ticket/{tar_id}``: ``{"status": "COMPLETE" | "UNKNOWN", "id": ...}``,
+  the status poll of the ColabFold client API (absent from server.py).
 * ``GET /result/download/{tar_id}``: that tar.gz, or 404.
 * ``POST /msa`` (JSON :class:`MSARequest`): one ndjson line,
   ``{"type": "data_msa", "data": {i: a3m}, "num_msas": n, "cache_folder": f}``,
@@ -99,6 +101,8 @@ def create_app(msa_fn: Callable | None = None, root: str | os.PathLike = ".",
     @app.post("/ticket/msa", include_in_schema=False)
     @app.post("/ticket/pair", include_in_schema=False)
     async def ticket_msa(request: Request):
+        # the ColabFold client also sends `mode` (env / all / pair...); as in
+        # server.py every mode searches the one database
         lines = (await form_q(request)).split("\n")
         num = int(os.getenv("MSAS_NUM", 4096))
         rank = int(os.getenv("MSAS_FFT_RANK", 4))
@@ -139,6 +143,14 @@ def create_app(msa_fn: Callable | None = None, root: str | os.PathLike = ".",
             tar.addfile(ti2, io.BytesIO(b""))
         result_store[tar_id] = tar_file
         return {"status": "COMPLETE", "id": tar_id}
+
+    @app.get("/ticket/{tar_id}", include_in_schema=False)
+    async def ticket_status(tar_id: str):
+        """Status poll of the ColabFold client API.  Searches finish before
+        the POST returns, so a known id is COMPLETE."""
+        path = result_store.get(tar_id)
+        status = "COMPLETE" if path is not None and path.exists() else "UNKNOWN"
+        return {"status": status, "id": tar_id}
 
     @app.get("/result/download/{tar_id}", include_in_schema=False)
     async def result_download(tar_id: str):

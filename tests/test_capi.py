@@ -40,7 +40,7 @@ def test_argument_errors_need_no_gpu():
     if not _lib.LIB_PATH.exists():
         pytest.skip("library not built")
     lib = _lib.load()
-    assert lib.launch_sw_cuda_affine(None, 2000, None, None, None, 5, None, 11, 1, None) == -1
+    assert lib.launch_sw_cuda_affine(None, 3000, None, None, None, 5, None, 11, 1, None) == -1
     assert b"query length" in lib.sseqs_last_error()
     assert lib.launch_sw_cuda_blosum62(None, 10, None, None, None, 5, None, 3, None) == -1
     assert lib.sseqs_topk(None, 0, 10, 11, 1, None, None, 0, None) == -1

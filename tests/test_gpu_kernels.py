@@ -15,6 +15,8 @@ from conftest import announce
 pytestmark = pytest.mark.gpu
 
 AA = "ARNDCQEGHILKMFPSTWYV"
+# what a packed database can hold: residues (weighted up) and other letters
+DB_ALPH = AA * 4 + "XBZUOJ"
 
 
 def rand_seq(rng, n, alphabet=AA):
@@ -69,6 +71,30 @@ def _packed_db(dev, text: bytes):
     return ops.compress(t)
 
 
+def test_from_xbit_files(dev, tmp_path):
+    """A database written as .xbit files (a main file cut at a chunk limit plus
+    an antibody-style second file, sseqs/msa.py:39-40,50-62) reads back into
+    the same index."""
+    from sseqs_amd import synth
+    from sseqs_amd.db import SeqDB
+    announce("from_xbit files")
+    db = SeqDB.synthetic(6000, [synth.make_query(100, 0, seed=2)], seed=5, device=dev)
+    raw = db.packed[:(db.n_syms * 5 + 7) // 8].cpu().numpy()
+    cb = 4000  # chunk size of this test; 3 chunks = 12000 bytes, a symbol boundary
+    main, ab = tmp_path / "main.xbit", tmp_path / "ab.xbit"
+    main.write_bytes(raw[:3 * cb + 123].tobytes())  # bytes past the chunk limit are ignored
+    ab.write_bytes(raw[3 * cb:].tobytes())
+    got = SeqDB.from_xbit(str(main), chunks=3, device=dev, ab_path=str(ab), ab_chunks=None,
+                          chunk_bytes=cb)
+    assert got.n == db.n
+    assert torch.equal(got.seq_off, db.seq_off) and torch.equal(got.seq_len, db.seq_len)
+    assert got.strings(got.seq_off.new_tensor([0, got.n - 1]).long(), 1000) == \
+        db.strings(db.seq_off.new_tensor([0, db.n - 1]).long(), 1000)
+    # the main file alone, whole
+    alone = SeqDB.from_xbit(str(main), device=dev)
+    assert alone.n > 0 and int(alone.seq_len.max()) <= 998
+
+
 def test_db_index_and_gather(orc, dev):
     from sseqs_amd.db import SeqDB
     announce("db_index")
@@ -97,7 +123,7 @@ def _batch(rng, query, n):
         if i % 3 == 0:
             tg.append(mutate(rng, query[int(rng.integers(0, max(1, len(query) // 2))):]))
         else:
-            tg.append(rand_seq(rng, rng.integers(1, 600)))
+            tg.append(rand_seq(rng, rng.integers(1, 600), AA * 4 + "XBZ*-"))
     text = "@".join(tg) + "@"
     a = np.frombuffer(text.encode(), dtype=np.uint8).copy()
     D = np.nonzero(a == 64)[0].astype(np.int32)
@@ -105,7 +131,7 @@ def _batch(rng, query, n):
 
 
 @pytest.mark.parametrize("Lq", [1, 2, 5, 17, 31, 32, 33, 40, 63, 64, 65, 100, 200, 257, 500,
-                                777, 1000, 1024])
+                                777, 1000, 1024, 1025, 2048])
 def test_sw_kernels_match_oracle(orc, dev, Lq):
     from sseqs_amd import ops
     announce(f"sw Lq={Lq}")
@@ -199,12 +225,29 @@ def test_db_prefilter_any_index_order(orc, dev, Lq, k):
     from sseqs_amd.search import prefilter_scores
     announce(f"db_prefilter unsorted Lq={Lq} k={k}")
     rng = np.random.default_rng(Lq)
-    query = rand_seq(rng, Lq)
-    seqs = [mutate(rng, query) if i % 5 == 0 else rand_seq(rng, rng.integers(1, 1100))
+    # letters a real .xbit file holds besides the 20 residues take the
+    # prefilter's code 20 (X) and 21 (everything else) paths
+    query = rand_seq(rng, Lq, AA * 3 + "X")
+    seqs = [mutate(rng, query) if i % 5 == 0 else rand_seq(rng, rng.integers(1, 1100), DB_ALPH)
             for i in range(2500)]
     text = ("@".join(seqs) + "@").encode()
     db = SeqDB.from_packed(_packed_db(dev, text), len(text))
     assert db.orig_id is not None
+    if k == 4:
+        # database-resident SW of some of them, affine and linear, bit-exact
+        from sseqs_amd.query import prepare as prep
+        from sseqs_amd.search import sw_affine_ids, sw_linear_ids
+        q4 = prep(query)
+        ids = torch.arange(0, db.n, 7, device=dev)
+        fid = db.orig_id.cpu().numpy()[ids.cpu().numpy()]
+        ascii = np.frombuffer(text, dtype=np.uint8)
+        starts = np.concatenate([[0], np.cumsum([len(x) + 1 for x in seqs])[:-1]])
+        lens = np.array([len(x) for x in seqs])
+        qc = orc.encode_query_sw(query)
+        assert np.array_equal(sw_affine_ids(db, q4, ids).cpu().numpy(),
+                              orc.sw_affine_segments(qc, ascii, starts[fid], lens[fid]))
+        assert np.array_equal(sw_linear_ids(db, q4, ids, -4).cpu().numpy(),
+                              orc.sw_linear_segments(qc, ascii, starts[fid], lens[fid], -4))
     got = prefilter_scores(db, prepare(query, fft_rank=k)).cpu().numpy()
     order = db.orig_id.cpu().numpy()
     A, B = orc.weighted_luts(query, k)

@@ -195,3 +195,27 @@ def test_server_ticket_roundtrip(orc, world, tmp_path, monkeypatch):
     assert got == "\n".join(rows)
     # one batch (bs = 1e8 bytes) keeps top_sw_affine = 2 hits
     assert got.count(">") == 3
+
+
+def test_msa_long_query_whole_hits(orc, world, tmp_path):
+    """A query past the reference's 1024-residue SW limit (prefilter N = 2048,
+    stripe width 47) with whole hits instead of 500-residue prefixes."""
+    import sseqs_amd
+    from sseqs_amd import synth
+    from sseqs_amd.query import prepare
+    from sseqs_amd.search import global_search
+    announce("msa long query")
+    db = world["db"]
+    query = synth.make_query(1500, 7, seed=4)
+    out = tmp_path / "long.a3m"
+    sseqs_amd.msa(query, str(out), num_msas=40, verbose=False, db=db, cascade="global",
+                  top_prefilter=400, hit_width=None)
+    gh = global_search(db, prepare(query), top_prefilter=400, top_hits=40)
+    cand = gh.candidates.cpu().numpy()
+    qc = orc.encode_query_sw(query)
+    sw = orc.sw_affine_segments(qc, world["ascii"], world["off"][cand], world["ln"][cand])
+    assert np.array_equal(gh.cand_scores.cpu().numpy(), sw)
+    seqs = db.strings(gh.ids, 1024)
+    assert max(len(s) for s in seqs) > 500 or all(len(s) <= 500 for s in seqs)
+    want = orc.a3m(query, [orc.backtrack(query, s) for s in seqs])
+    assert out.read_text() == want

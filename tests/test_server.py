@@ -134,3 +134,11 @@ def test_ticket_requires_q(stub_app):
     client, stub, _ = stub_app
     assert client.post("/ticket/msa", data={"x": "1"}).status_code == 422
     assert not stub.calls
+
+
+def test_ticket_status_poll(stub_app):
+    client, _, _ = stub_app
+    r = client.post("/ticket/msa", data={"q": ">a\nMKTAYIAK", "mode": "env"})
+    tid = r.json()["id"]
+    assert client.get(f"/ticket/{tid}").json() == {"status": "COMPLETE", "id": tid}
+    assert client.get("/ticket/nope").json() == {"status": "UNKNOWN", "id": "nope"}
