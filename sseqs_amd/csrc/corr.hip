@@ -574,6 +574,35 @@ __global__ __launch_bounds__(64) void k_corr_db4(const uint8_t* __restrict__ pac
 // ---------------------------------------------------------------------------
 __device__ inline float max4(f32x4 v) { return fmaxf(fmaxf(v[0], v[1]), fmaxf(v[2], v[3])); }
 
+// Reductions over the 16 lanes of a DPP row (lanes 16r .. 16r+15) by
+// rotations row_ror:8, 4, 2, 1; afterwards every lane of the row holds the
+// result.
+template <int CTRL>
+__device__ inline int dpp_row(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ inline int64_t dpp_row64(int64_t v) {
+    const int lo = dpp_row<CTRL>((int)(uint32_t)v), hi = dpp_row<CTRL>((int)(v >> 32));
+    return (int64_t)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+__device__ inline int row_max(int v) {
+    v = max(v, dpp_row<0x128>(v));
+    v = max(v, dpp_row<0x124>(v));
+    v = max(v, dpp_row<0x122>(v));
+    v = max(v, dpp_row<0x121>(v));
+    return v;
+}
+template <bool MAX>
+__device__ inline int64_t row_ext64(int64_t v) {
+    int64_t o;
+    o = dpp_row64<0x128>(v); v = MAX ? (o > v ? o : v) : (o < v ? o : v);
+    o = dpp_row64<0x124>(v); v = MAX ? (o > v ? o : v) : (o < v ? o : v);
+    o = dpp_row64<0x122>(v); v = MAX ? (o > v ? o : v) : (o < v ? o : v);
+    o = dpp_row64<0x121>(v); v = MAX ? (o > v ? o : v) : (o < v ? o : v);
+    return v;
+}
+
 template <int R, bool FULL, bool DBL>
 __global__ __launch_bounds__(64, (R <= 8 ? 2 : 1)) void k_corr_band(const uint8_t* __restrict__ packed,
                                                   const int64_t* __restrict__ off,
@@ -668,15 +697,11 @@ __global__ __launch_bounds__(64, (R <= 8 ? 2 : 1)) void k_corr_band(const uint8_
         Tile T;
         T.in = t * 16 + col < nseq;
         const int Le = T.in ? max(0, min(min(L, maxlen), N)) : 0;
-        int kp = Le;
-        int64_t lo = T.in ? o : INT64_MAX, hi = T.in ? o : 0, ehi = T.in ? o + Le : 0;
-#pragma unroll
-        for (int d = 8; d > 0; d >>= 1) {
-            kp = max(kp, __shfl_xor(kp, d, 64));
-            lo = min(lo, __shfl_xor(lo, d, 64));
-            hi = max(hi, __shfl_xor(hi, d, 64));
-            ehi = max(ehi, __shfl_xor(ehi, d, 64));
-        }
+        // the 16 columns of the tile are the 16 lanes of every DPP row
+        const int kp = row_max(Le);
+        const int64_t lo = row_ext64<false>(T.in ? o : INT64_MAX);
+        const int64_t hi = row_ext64<true>(T.in ? o : 0);
+        const int64_t ehi = row_ext64<true>(T.in ? o + Le : 0);
         T.C = __builtin_amdgcn_readfirstlane((kp + 31) >> 5);  // chunks holding residues, <= ND
         T.o = T.in ? o : lo;  // idle lanes decode (and mask) inside the stage
         T.Le = Le;

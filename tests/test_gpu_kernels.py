@@ -169,7 +169,7 @@ def test_sw_edge_cases(orc, dev):
     empty = ops.sw_cuda_affine(t(np.zeros(0, np.uint8)), t(gi), t(a), t(P)).cpu().numpy()
     assert empty.tolist() == [-1, -1, -1, -1]
     with pytest.raises(RuntimeError):
-        ops.sw_cuda_affine(t(np.zeros(1025, np.uint8)), t(gi), t(a), t(P))
+        ops.sw_cuda_affine(t(np.zeros(2049, np.uint8)), t(gi), t(a), t(P))
     with pytest.raises(RuntimeError):
         ops.sw_cuda_blosum62(t(q), t(gi), t(a), t(P), gap_penalty_param=2)
     with pytest.raises(TypeError):

@@ -216,6 +216,7 @@ def test_msa_long_query_whole_hits(orc, world, tmp_path):
     sw = orc.sw_affine_segments(qc, world["ascii"], world["off"][cand], world["ln"][cand])
     assert np.array_equal(gh.cand_scores.cpu().numpy(), sw)
     seqs = db.strings(gh.ids, 1024)
-    assert max(len(s) for s in seqs) > 500 or all(len(s) <= 500 for s in seqs)
+    # whole hits: every row is the full database sequence
+    assert [len(x) for x in seqs] == db.seq_len[gh.ids].cpu().tolist()
     want = orc.a3m(query, [orc.backtrack(query, s) for s in seqs])
     assert out.read_text() == want
