@@ -697,11 +697,29 @@ __global__ __launch_bounds__(64, (R <= 8 ? 2 : 1)) void k_corr_band(const uint8_
         Tile T;
         T.in = t * 16 + col < nseq;
         const int Le = T.in ? max(0, min(min(L, maxlen), N)) : 0;
-        // the 16 columns of the tile are the 16 lanes of every DPP row
-        const int kp = row_max(Le);
-        const int64_t lo = row_ext64<false>(T.in ? o : INT64_MAX);
-        const int64_t hi = row_ext64<true>(T.in ? o : 0);
-        const int64_t ehi = row_ext64<true>(T.in ? o + Le : 0);
+        // Extent of the tile (the 16 columns are the 16 lanes of every DPP
+        // row).  In a length-sorted stream offsets and lengths rise across
+        // the tile, so the first and last sequences bound it; otherwise
+        // reduce over the row.
+        const int nin = (int)min((int64_t)16, nseq - t * 16);  // sequences in the tile
+        const int64_t o_prev = dpp_row64<0x111>(o);            // row_shr:1, column col-1
+        const int L_prev = dpp_row<0x111>(Le);
+        const bool rising = col == 0 || col >= nin || (o > o_prev && Le >= L_prev);
+        int kp;
+        int64_t lo, hi, ehi;
+        if (__ballot(!rising) == 0) {
+            kp = __builtin_amdgcn_readlane(Le, nin - 1);
+            lo = ((int64_t)__builtin_amdgcn_readfirstlane((int)(o >> 32)) << 32) |
+                 (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)o);
+            hi = ((int64_t)__builtin_amdgcn_readlane((int)(o >> 32), nin - 1) << 32) |
+                 (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)o, nin - 1);
+            ehi = hi + kp;
+        } else {
+            kp = row_max(Le);
+            lo = row_ext64<false>(T.in ? o : INT64_MAX);
+            hi = row_ext64<true>(T.in ? o : 0);
+            ehi = row_ext64<true>(T.in ? o + Le : 0);
+        }
         T.C = __builtin_amdgcn_readfirstlane((kp + 31) >> 5);  // chunks holding residues, <= ND
         T.o = T.in ? o : lo;  // idle lanes decode (and mask) inside the stage
         T.Le = Le;
