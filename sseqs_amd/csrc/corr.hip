@@ -603,22 +603,21 @@ __device__ inline int64_t row_ext64(int64_t v) {
     return v;
 }
 
-template <int R, bool FULL, bool DBL>
-__global__ __launch_bounds__(64, (R <= 8 ? 2 : 1)) void k_corr_band(const uint8_t* __restrict__ packed,
-                                                  const int64_t* __restrict__ off,
-                                                  const int32_t* __restrict__ len, int64_t s_begin,
-                                                  int64_t nseq, const uint8_t* __restrict__ qrev,
-                                                  int Lq, const float* __restrict__ lutA,
-                                                  const float* __restrict__ lutB, int V, int N,
-                                                  int maxlen, int CAPB, int d0, int Rn,
-                                                  float* __restrict__ out) {
+template <int R, bool FULL, bool DBL, int G>
+__global__ __launch_bounds__(64, (G == 1 && R <= 8 ? 2 : 1)) void k_corr_band(
+    const uint8_t* __restrict__ packed, const int64_t* __restrict__ off,
+    const int32_t* __restrict__ len, int64_t s_begin, int64_t nseq,
+    const uint8_t* __restrict__ qrev, int Lq, const float* __restrict__ lutA,
+    const float* __restrict__ lutB, int V, int N, int maxlen, int CAPB, int d0, int Rn,
+    float* __restrict__ out) {
     constexpr int K = 4;
+    constexpr int TS = 16 * G;  // sequences per tile: G groups of 16 columns
     extern __shared__ __align__(16) uint8_t smem[];
     uint64_t* btab4 = reinterpret_cast<uint64_t*>(smem);  // [40] x 4 fp16
     uint8_t* region = smem + CORR_NB * 8;
     uint64_t* qtab4 = reinterpret_cast<uint64_t*>(region);  // [N+8] x 4 fp16, set-up only
-    f32x4* save = reinterpret_cast<f32x4*>(region);         // [R][64]
-    uint8_t* stg = region + (FULL ? 0 : R * 64 * 16);
+    f32x4* save = reinterpret_cast<f32x4*>(region);         // [G][R][64]
+    uint8_t* stg = region + (FULL ? 0 : G * R * 64 * 16);
 
     const int lane = threadIdx.x;
     const int Lqe = min(Lq, N);
@@ -649,7 +648,8 @@ __global__ __launch_bounds__(64, (R <= 8 ? 2 : 1)) void k_corr_band(const uint8_
     }
     __syncthreads();
 
-    // lane (col, g): A row col, B/C column col, positions 8g..8g+7 of a chunk
+    // lane (col, g): A row col, B/C column col (of every group), positions
+    // 8g..8g+7 of a chunk
     const int col = lane & 15, g = lane >> 4;
     // K is (position, rank) interleaved: element x of lane (col, g) in
     // sub-chunk q of chunk c is position 32c + 8g + 2q + (x >> 2), rank x & 3.
@@ -673,7 +673,7 @@ __global__ __launch_bounds__(64, (R <= 8 ? 2 : 1)) void k_corr_band(const uint8_
     __syncthreads();  // the region now holds save + stage
 
     const int ND = N >> 5;
-    const int64_t ntiles = (nseq + 15) >> 4;
+    const int64_t ntiles = (nseq + TS - 1) / TS;
     int64_t tile = blockIdx.x;
     if (tile >= ntiles) return;
     // stage buffers: the next tile's bytes land in one (asynchronous
@@ -683,46 +683,65 @@ __global__ __launch_bounds__(64, (R <= 8 ? 2 : 1)) void k_corr_band(const uint8_
 
     // raw index entries of tile t's sequences (clamped into range, so the
     // loads are unconditional and nothing waits for them before use)
-    auto meta = [&](int64_t t, int64_t& o, int& L) {
-        const int64_t sid = min(t * 16 + col, nseq - 1);
-        o = off[s_begin + sid];
-        L = len[s_begin + sid];
+    auto meta = [&](int64_t t, int64_t* o, int* L) {
+#pragma unroll
+        for (int gg = 0; gg < G; ++gg) {
+            const int64_t sid = min(t * TS + 16 * gg + col, nseq - 1);
+            o[gg] = off[s_begin + sid];
+            L[gg] = len[s_begin + sid];
+        }
     };
     struct Tile {
-        int64_t o, b_lo, nchunk;
-        int Le, C;
-        bool in, fast;
+        int64_t o[G], b_lo, nchunk;
+        int Le[G], C;
+        bool in[G], fast;
     };
-    auto info = [&](int64_t t, int64_t o, int L) -> Tile {
+    auto info = [&](int64_t t, const int64_t* o, const int* L) -> Tile {
         Tile T;
-        T.in = t * 16 + col < nseq;
-        const int Le = T.in ? max(0, min(min(L, maxlen), N)) : 0;
+        int kp = 0;
+        int64_t lo = INT64_MAX, hi = 0, ehi = 0;
+        // this lane's sequences (one per group)
+#pragma unroll
+        for (int gg = 0; gg < G; ++gg) {
+            T.in[gg] = t * TS + 16 * gg + col < nseq;
+            T.Le[gg] = T.in[gg] ? max(0, min(min(L[gg], maxlen), N)) : 0;
+            if (T.in[gg]) {
+                kp = max(kp, T.Le[gg]);
+                lo = min(lo, o[gg]);
+                hi = max(hi, o[gg]);
+                ehi = max(ehi, o[gg] + T.Le[gg]);
+            }
+        }
         // Extent of the tile (the 16 columns are the 16 lanes of every DPP
         // row).  In a length-sorted stream offsets and lengths rise across
-        // the tile, so the first and last sequences bound it; otherwise
-        // reduce over the row.
-        const int nin = (int)min((int64_t)16, nseq - t * 16);  // sequences in the tile
-        const int64_t o_prev = dpp_row64<0x111>(o);            // row_shr:1, column col-1
-        const int L_prev = dpp_row<0x111>(Le);
-        const bool rising = col == 0 || col >= nin || (o > o_prev && Le >= L_prev);
-        int kp;
-        int64_t lo, hi, ehi;
-        if (__ballot(!rising) == 0) {
-            kp = __builtin_amdgcn_readlane(Le, nin - 1);
-            lo = ((int64_t)__builtin_amdgcn_readfirstlane((int)(o >> 32)) << 32) |
-                 (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)o);
-            hi = ((int64_t)__builtin_amdgcn_readlane((int)(o >> 32), nin - 1) << 32) |
-                 (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)o, nin - 1);
+        // a one-group tile, so its first and last sequences bound it;
+        // otherwise reduce over the row.
+        bool rising = false;
+        int nin = 0;
+        if (G == 1) {
+            nin = (int)min((int64_t)16, nseq - t * 16);
+            const int64_t o_prev = dpp_row64<0x111>(o[0]);  // row_shr:1, column col-1
+            const int L_prev = dpp_row<0x111>(T.Le[0]);
+            rising = __ballot(!(col == 0 || col >= nin ||
+                                (o[0] > o_prev && T.Le[0] >= L_prev))) == 0;
+        }
+        if (rising) {
+            kp = __builtin_amdgcn_readlane(T.Le[0], nin - 1);
+            lo = ((int64_t)__builtin_amdgcn_readfirstlane((int)(o[0] >> 32)) << 32) |
+                 (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)o[0]);
+            hi = ((int64_t)__builtin_amdgcn_readlane((int)(o[0] >> 32), nin - 1) << 32) |
+                 (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)o[0], nin - 1);
             ehi = hi + kp;
         } else {
-            kp = row_max(Le);
-            lo = row_ext64<false>(T.in ? o : INT64_MAX);
-            hi = row_ext64<true>(T.in ? o : 0);
-            ehi = row_ext64<true>(T.in ? o + Le : 0);
+            kp = row_max(kp);
+            lo = row_ext64<false>(lo);
+            hi = row_ext64<true>(hi);
+            ehi = row_ext64<true>(ehi);
         }
         T.C = __builtin_amdgcn_readfirstlane((kp + 31) >> 5);  // chunks holding residues, <= ND
-        T.o = T.in ? o : lo;  // idle lanes decode (and mask) inside the stage
-        T.Le = Le;
+#pragma unroll
+        for (int gg = 0; gg < G; ++gg)
+            T.o[gg] = T.in[gg] ? o[gg] : lo;  // idle lanes decode (and mask) inside the stage
         T.b_lo = ((lo * 5) >> 3) & ~(int64_t)15;
         // the last window of a lane starts at byte 5*(o+32C-8)/8 and is 16 bytes
         T.fast = ((((hi + 32 * T.C) * 5) >> 3) + 24 - T.b_lo) <= CAPB;
@@ -747,15 +766,25 @@ __global__ __launch_bounds__(64, (R <= 8 ? 2 : 1)) void k_corr_band(const uint8_
             }
     };
 
-    int64_t o_n = 0, o_c = 0;
-    int Le_n = 0, Le_c = 0;
+    int64_t o_n[G], o_c[G];
+    int Le_n[G], Le_c[G];
+#pragma unroll
+    for (int gg = 0; gg < G; ++gg) {
+        o_n[gg] = 0;
+        Le_n[gg] = 0;
+    }
     meta(tile, o_c, Le_c);
     if (tile + gridDim.x < ntiles) meta(tile + gridDim.x, o_n, Le_n);
     Tile cur = info(tile, o_c, Le_c);
     issue(cur, stg);
     int sb = 0;
-    int64_t prev_sid = -1;  // this lane's result of the previous tile, stored late
-    float prev_best = 0.f;
+    int64_t prev_sid[G];  // this lane's results of the previous tile, stored late
+    float prev_best[G];
+#pragma unroll
+    for (int gg = 0; gg < G; ++gg) {
+        prev_sid[gg] = -1;
+        prev_best[gg] = 0.f;
+    }
     for (;;) {
         // the current tile's bytes (and the next tile's index entries) are in
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -767,19 +796,23 @@ __global__ __launch_bounds__(64, (R <= 8 ? 2 : 1)) void k_corr_band(const uint8_
             if (DBL) issue(nxt, stg + (sb ^ 1) * SB2);
         }
         if (tn + gridDim.x < ntiles) meta(tn + gridDim.x, o_n, Le_n);
-        if (prev_sid >= 0) out[prev_sid] = prev_best;
+#pragma unroll
+        for (int gg = 0; gg < G; ++gg)
+            if (prev_sid[gg] >= 0) out[prev_sid[gg]] = prev_best[gg];
         const uint8_t* sbuf = stg + sb * SB2;
-        const int64_t o = cur.o;
-        const int Le = cur.Le, C = cur.C;
-        const int64_t relbit = 5 * o - 8 * cur.b_lo;
+        const int C = cur.C;
+        int64_t relbit[G];  // bit of each sequence of this lane in the stage
+#pragma unroll
+        for (int gg = 0; gg < G; ++gg) relbit[gg] = 5 * cur.o[gg] - 8 * cur.b_lo;
 
-        // B fragments (4 sub-chunks) of chunk v: this lane's 8 positions
-        // 32v+8g.., symbol 0 past the sequence's end
-        auto decode = [&](auto fast_tag, int v, half8* b) {
+        // B fragments (4 sub-chunks) of chunk v of group gg: this lane's 8
+        // positions 32v+8g.., symbol 0 past the sequence's end
+        auto decode = [&](auto fast_tag, int gg, int v, half8* b) {
             const int pb = 32 * v + 8 * g;
+            const int Le = cur.Le[gg];
             uint32_t sym[8];
             if constexpr (decltype(fast_tag)::value) {
-                const int64_t bit = relbit + 5 * (int64_t)pb;
+                const int64_t bit = relbit[gg] + 5 * (int64_t)pb;
                 const int64_t byte = bit >> 3;
                 const uint64_t* s64 = reinterpret_cast<const uint64_t*>(sbuf + (byte & ~7));
                 const uint64_t B0 = __builtin_bswap64(s64[0]);
@@ -794,7 +827,7 @@ __global__ __launch_bounds__(64, (R <= 8 ? 2 : 1)) void k_corr_band(const uint8_
             } else {
 #pragma unroll
                 for (int e = 0; e < 8; ++e)
-                    sym[e] = pb + e < Le ? packed_sym(packed, o + pb + e) - 64u : 0u;
+                    sym[e] = pb + e < Le ? packed_sym(packed, cur.o[gg] + pb + e) - 64u : 0u;
             }
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -810,67 +843,90 @@ __global__ __launch_bounds__(64, (R <= 8 ? 2 : 1)) void k_corr_band(const uint8_
         // The tile's chunks; chunk v+1 is decoded in the same basic block as
         // chunk v's MFMAs, so the two interleave.  Two instances: symbols
         // from the stage (fast) or from global memory.
-        auto run = [&](auto fast_tag) -> float {
-            f32x4 acc[R];
+        auto run = [&](auto fast_tag, float* best) {
+            f32x4 acc[G][R];
 #pragma unroll
-            for (int t = 0; t < R; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-            float best = -INFINITY;
+            for (int gg = 0; gg < G; ++gg) {
+                best[gg] = -INFINITY;
+#pragma unroll
+                for (int t = 0; t < R; ++t) acc[gg][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+            }
             const int x = FULL ? 0 : max(0, C - ND + Rn - 1);
             // B fragments of the current and the next chunk alternate between
             // bb[0] and bb[1] (R is even, so chunk v always uses bb[v & 1])
             static_assert(R % 2 == 0, "ring size must be even");
-            half8 bb[2][4];
-            if (C > 0) decode(fast_tag, 0, bb[0]);
+            half8 bb[2][G][4];
+            if (C > 0)
+#pragma unroll
+                for (int gg = 0; gg < G; ++gg) decode(fast_tag, gg, 0, bb[0][gg]);
             for (int vb = 0; vb < C; vb += R) {
 #pragma unroll
                 for (int s = 0; s < R; ++s) {
                     const int v = vb + s;
                     if (v < C) {
-                        half8* bc = bb[s & 1];
-                        decode(fast_tag, min(v + 1, C - 1), bb[(s + 1) & 1]);
+#pragma unroll
+                        for (int gg = 0; gg < G; ++gg)
+                            decode(fast_tag, gg, min(v + 1, C - 1), bb[(s + 1) & 1][gg]);
                         // consecutive MFMAs go to different accumulators
 #pragma unroll
                         for (int q = 0; q < 4; ++q)
 #pragma unroll
                             for (int j = 0; j < R; ++j)
-                                acc[(s + j) % R] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                                    A[j][q], bc[q], acc[(s + j) % R], 0, 0, 0);
+#pragma unroll
+                                for (int gg = 0; gg < G; ++gg)
+                                    acc[gg][(s + j) % R] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                                        A[j][q], bb[s & 1][gg][q], acc[gg][(s + j) % R], 0, 0,
+                                        0);
                         if (!FULL) {
                             // relative m-tile v is complete: park it (slot R-1
                             // takes the stores of the ones that are not)
                             const bool park = v < x;
-                            save[(park ? v : R - 1) * 64 + lane] = acc[s];
-                            const float m = max4(acc[s]);
-                            best = park ? best : fmaxf(best, m);
-                            acc[s] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                            for (int gg = 0; gg < G; ++gg) {
+                                save[((gg * R) + (park ? v : R - 1)) * 64 + lane] = acc[gg][s];
+                                const float m = max4(acc[gg][s]);
+                                best[gg] = park ? best[gg] : fmaxf(best[gg], m);
+                                acc[gg][s] = f32x4{0.f, 0.f, 0.f, 0.f};
+                            }
                         }
                     }
                 }
             }
-            if (FULL) {
 #pragma unroll
-                for (int t = 0; t < R; ++t) best = fmaxf(best, max4(acc[t]));
-            } else {
-                // live: rel = C .. C+R-2 in slots rel mod R
+            for (int gg = 0; gg < G; ++gg) {
+                if (FULL) {
 #pragma unroll
-                for (int t = 0; t < R; ++t) {
-                    const int rel = C + ((t - C) % R + R) % R;
-                    if (rel == C + R - 1) continue;
-                    if (rel < ND)
-                        best = fmaxf(best, max4(acc[t]));
-                    else if (rel - ND < x)
-                        best = fmaxf(best, max4(acc[t] + save[(rel - ND) * 64 + lane]));
+                    for (int t = 0; t < R; ++t) best[gg] = fmaxf(best[gg], max4(acc[gg][t]));
+                } else {
+                    // live: rel = C .. C+R-2 in slots rel mod R
+#pragma unroll
+                    for (int t = 0; t < R; ++t) {
+                        const int rel = C + ((t - C) % R + R) % R;
+                        if (rel == C + R - 1) continue;
+                        if (rel < ND)
+                            best[gg] = fmaxf(best[gg], max4(acc[gg][t]));
+                        else if (rel - ND < x)
+                            best[gg] = fmaxf(best[gg],
+                                             max4(acc[gg][t] +
+                                                  save[(gg * R + rel - ND) * 64 + lane]));
+                    }
+                    if (C + R - 1 < ND) best[gg] = fmaxf(best[gg], 0.f);  // unreached m-tiles
                 }
-                if (C + R - 1 < ND) best = fmaxf(best, 0.f);  // m-tiles no chunk reaches
             }
-            return best;
         };
-        float best = cur.fast ? run(std::true_type{}) : run(std::false_type{});
-        // the four lane groups hold different rows of the same columns
-        best = fmaxf(best, __shfl_xor(best, 16, 64));
-        best = fmaxf(best, __shfl_xor(best, 32, 64));
-        prev_best = best;
-        prev_sid = (g == 0 && cur.in) ? tile * 16 + col : -1;
+        float best[G];
+        if (cur.fast)
+            run(std::true_type{}, best);
+        else
+            run(std::false_type{}, best);
+#pragma unroll
+        for (int gg = 0; gg < G; ++gg) {
+            // the four lane groups hold different rows of the same columns
+            float bm = fmaxf(best[gg], __shfl_xor(best[gg], 16, 64));
+            bm = fmaxf(bm, __shfl_xor(bm, 32, 64));
+            prev_best[gg] = bm;
+            prev_sid[gg] = (g == 0 && cur.in[gg]) ? tile * TS + 16 * gg + col : -1;
+        }
 
         tile = tn;
         if (tile >= ntiles) break;
@@ -882,7 +938,9 @@ __global__ __launch_bounds__(64, (R <= 8 ? 2 : 1)) void k_corr_band(const uint8_
         }
         cur = nxt;
     }
-    if (prev_sid >= 0) out[prev_sid] = prev_best;
+#pragma unroll
+    for (int gg = 0; gg < G; ++gg)
+        if (prev_sid[gg] >= 0) out[prev_sid[gg]] = prev_best[gg];
 }
 
 // Nonzero diagonals of the band kernel: diagonal d's A blocks read lags
@@ -952,6 +1010,7 @@ extern "C" int launch_warp_corr(uint16_t* corr, const uint8_t* qrev, const float
 struct PrefilterPlan {
     enum Kind { BAND, DB4, DB } kind;
     int R = 0;        // band: ring size (diagonals held in registers)
+    int G = 1;        // band: groups of 16 sequences per tile
     bool full = false, dbl = false;
     int d0 = 0, rn = 0;
     int nmt = 1;      // dense kernels: m-tiles per pass
@@ -975,19 +1034,22 @@ static PrefilterPlan plan_prefilter(int query_len, int k, int N, int mlen) {
         }
         if (P.R) {
             P.kind = PrefilterPlan::BAND;
-            P.tile = 16;
-            // a tile of 16 length-sorted sequences spans about 16 of them
+            // one group of 16 sequences per tile: two groups per tile at one
+            // wave per SIMD measured 27 % slower than one group at two waves
+            P.G = 1;
+            P.tile = 16 * P.G;
+            // a tile of length-sorted sequences spans about that many of them
             const int span = (min(mlen, N) + 31) & ~31;
-            P.capb = (int)((((int64_t)16 * (span + 1) * 5) / 8 + 48 + 15) & ~15);
+            P.capb = (int)((((int64_t)P.tile * (span + 1) * 5) / 8 + 48 + 15) & ~15);
             const size_t qt = (size_t)4 * (N + 8) * 2;
             const size_t sbytes = (size_t)((P.capb + 64 + 15) & ~15);
-            const size_t save = P.full ? 0 : (size_t)P.R * 64 * 16;
+            const size_t save = P.full ? 0 : (size_t)P.G * P.R * 64 * 16;
             auto lds_for = [&](int nbuf) {
                 const size_t run = save + nbuf * sbytes;
                 return ((size_t)CORR_NB * 8 + (qt > run ? qt : run) + 15) & ~(size_t)15;
             };
-            // two stage buffers while 8 blocks (2 waves per SIMD) still fit a CU
-            P.dbl = lds_for(2) <= 20 * 1024;
+            // two stage buffers while the blocks of 8 waves per CU still fit
+            P.dbl = lds_for(2) <= (P.G == 1 ? 20 : 40) * 1024;
             if (const char* e = getenv("SSEQS_CORR_DBL")) P.dbl = atoi(e) != 0;
             P.lds = lds_for(P.dbl ? 2 : 1);
             return P;
@@ -1016,8 +1078,8 @@ static PrefilterPlan plan_prefilter(int query_len, int k, int N, int mlen) {
 static std::string plan_name(const PrefilterPlan& P) {
     char b[64];
     if (P.kind == PrefilterPlan::BAND)
-        snprintf(b, sizeof b, "k_corr_band<%d, %s, %s>", P.R, P.full ? "true" : "false",
-                 P.dbl ? "true" : "false");
+        snprintf(b, sizeof b, "k_corr_band<%d, %s, %s, %d>", P.R, P.full ? "true" : "false",
+                 P.dbl ? "true" : "false", P.G);
     else
         snprintf(b, sizeof b, "%s<%d>", P.kind == PrefilterPlan::DB4 ? "k_corr_db4" : "k_corr_db",
                  P.nmt);
@@ -1069,16 +1131,16 @@ extern "C" int sseqs_db_prefilter(const uint8_t* packed, const int64_t* seq_off,
         auto pick = [&](auto dbl_tag) -> int {
             constexpr bool D = decltype(dbl_tag)::value;
             if (P.full)
-                return P.R == 4   ? gob(&k_corr_band<4, true, D>)
-                       : P.R == 8 ? gob(&k_corr_band<8, true, D>)
-                                  : gob(&k_corr_band<16, true, D>);
+                return P.R == 4   ? gob(&k_corr_band<4, true, D, 1>)
+                       : P.R == 8 ? gob(&k_corr_band<8, true, D, 1>)
+                                  : gob(&k_corr_band<16, true, D, 1>);
             switch (P.R) {
-            case 4: return gob(&k_corr_band<4, false, D>);
-            case 6: return gob(&k_corr_band<6, false, D>);
-            case 8: return gob(&k_corr_band<8, false, D>);
-            case 12: return gob(&k_corr_band<12, false, D>);
-            case 16: return gob(&k_corr_band<16, false, D>);
-            default: return gob(&k_corr_band<18, false, D>);
+            case 4: return gob(&k_corr_band<4, false, D, 1>);
+            case 6: return gob(&k_corr_band<6, false, D, 1>);
+            case 8: return gob(&k_corr_band<8, false, D, 1>);
+            case 12: return gob(&k_corr_band<12, false, D, 1>);
+            case 16: return gob(&k_corr_band<16, false, D, 1>);
+            default: return gob(&k_corr_band<18, false, D, 1>);
             }
         };
         return P.dbl ? pick(std::true_type{}) : pick(std::false_type{});

@@ -62,10 +62,10 @@ def test_prefilter_kernel_plan_needs_no_gpu():
         return buf.value.decode()
 
     # q_len 200: every one of the 8 diagonals is nonzero at N = 256, 8 of 16 at 512
-    assert name(200, 4, 256) == "k_corr_band<8, true, true>"
-    assert name(200, 4, 512) == "k_corr_band<8, false, true>"
-    assert name(200, 4, 1024) == "k_corr_band<8, false, false>"
-    assert name(500, 4, 512) == "k_corr_band<16, true, true>"
+    assert name(200, 4, 256) == "k_corr_band<8, true, true, 1>"
+    assert name(200, 4, 512) == "k_corr_band<8, false, true, 1>"
+    assert name(200, 4, 1024) == "k_corr_band<8, false, false, 1>"
+    assert name(500, 4, 512) == "k_corr_band<16, true, true, 1>"
     assert name(500, 4, 1024).startswith("k_corr_band<18, false")
     assert name(200, 3, 256) == "k_corr_db<4>"
     assert name(50, 4, 64) == "k_corr_db4<1>"
