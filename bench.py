@@ -125,6 +125,8 @@ def main() -> int:
     ap.add_argument("--top-hits", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-q500", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=2,
+                    help="timed passes of the per-batch cascade (0 skips it)")
     ap.add_argument("--prefilter-only", action="store_true",
                     help="run only the q_len=200 prefilter sweep --steps times (for counters)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -260,7 +262,30 @@ def main() -> int:
         log(json.dumps(res))
         return res, launches
 
+    def measure_batch(query: str, bs: int = 50_000_000):
+        """The per-batch cascade that msa() runs by default (sseqs/msa.py:196-370):
+        batches of bs packed bytes, each unpacked, segmented, prefiltered and
+        cascaded to 2 hits, with the host reading every batch's hits."""
+        from sseqs_amd.search import batch_search
+        q = prepare(query, device=dev)
+        hits = batch_search(db, query, q, bs=bs)
+        torch.cuda.synchronize()
+        log(f"batch cascade warmup q_len={len(query)}")
+        t_wall = time.perf_counter()
+        for i in range(args.batch_steps):
+            hits = batch_search(db, query, q, bs=bs)
+        torch.cuda.synchronize()
+        sec = (time.perf_counter() - t_wall) / args.batch_steps
+        res = {"q_len": len(query), "bs": bs, "batches": -(-db.nbytes // bs),
+               "passes": args.batch_steps, "ms_per_pass": round(1e3 * sec, 1),
+               "db_gb_per_s": round(db.nbytes / sec / 1e9, 2),
+               "search_gcups": round(len(query) * db.residues / sec / 1e9, 1),
+               "hits": len(hits), "best_norm": round(max(h.norm for h in hits), 3)}
+        log(json.dumps(res))
+        return res
+
     r200, launches200 = measure(q200)
+    rbatch = measure_batch(q200) if args.batch_steps > 0 else None
     r500 = r500l = None
     if not args.no_q500:
         r500, _ = measure(q500)
@@ -318,6 +343,7 @@ def main() -> int:
                          "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBPS, 4),
                          "kernel": dom["kernel"],
                          "algorithmic": "packed bytes 5/8*(L+1) per sequence of the launch"},
+        "msa_batch_cascade": rbatch,
         "q500": r500,
         "q500_linear4": r500l,
         "cpu_baseline": cpu,

@@ -42,7 +42,7 @@ def msa(queries, filenames=None, num_msas=4096, save_raw="", verbose=True, bs=50
             with torch.cuda.device(gpu):
                 msa(queries, filenames, num_msas, save_raw, verbose, bs, fft_rank, top_fft,
                     top_sw, top_sw_affine, return_timings, sync_time, dlength, gpu, True,
-                    db=db, cascade=cascade, top_prefilter=top_prefilter)
+                    db=db, cascade=cascade, top_prefilter=top_prefilter, hit_width=hit_width)
 
         nt = threading.Thread(target=_target)
         _last[gpu] = nt
