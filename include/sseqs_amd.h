@@ -166,6 +166,21 @@ size_t sseqs_topk_ws_bytes(int64_t n);
 int sseqs_topk(const void* scores, int dtype, int64_t n, int64_t k, int sorted, int64_t* out_idx,
                void* ws, size_t ws_bytes, hipStream_t stream);
 
+/* sseqs_topk over many segments in one launch: the per-batch selects of the
+ * reference cascade (sseqs/msa.py:297-310, 324, 335) for all batches of a
+ * sweep.  Segment j holds scores[seg_begin[j] .. + seg_n[j]) and selects its
+ * seg_k[j] largest (ties to the smaller index; sorted != 0: ordered by
+ * (score desc, index asc), seg_k <= 4096; sorted == 0: ascending index).
+ * Entries drop_front .. seg_k[j] - drop_back - 1 of that order are written
+ * to out_idx[out_base[j] ..], as indices within the segment, plus
+ * seg_begin[j] when add_begin != 0.  max_k bounds every seg_k[j] (checked
+ * against the sorted limit on the host).  Device arrays: seg_begin,
+ * out_base (int64), seg_n, seg_k (int32). */
+int sseqs_topk_segments(const void* scores, int dtype, const int64_t* seg_begin,
+                        const int32_t* seg_n, const int32_t* seg_k, const int64_t* out_base,
+                        int64_t nseg, int max_k, int sorted, int drop_front, int drop_back,
+                        int add_begin, int64_t* out_idx, hipStream_t stream);
+
 /* ------------------------------------------------------------ backtrack -- */
 
 /* Full affine DP (gaps open from H, -5 for non-residues) and stateless

@@ -221,6 +221,162 @@ __global__ __launch_bounds__(1024) void k_sel_sort(const void* s, int dtype, con
     for (int i = threadIdx.x; i < k; i += 1024) out[i] = (int64_t)(uint32_t)comp[i];
 }
 
+// ------------------------------------------------------------------------
+// Segmented top-k: the per-batch selects of the reference's cascade
+// (sseqs/msa.py:297-310, 324, 335) for every batch of a sweep in one launch.
+// One block of SEG_THREADS threads per segment: histogram passes over the
+// segment's keys in LDS fix the k-th largest key T, then one pass in index
+// order writes the selection (keys > T, then the first keys == T), or the
+// selection is staged in LDS and sorted by (key desc, index asc).
+
+constexpr int SEG_THREADS = 1024;
+constexpr int SEG_WAVES = SEG_THREADS / 64;
+constexpr int SEG_SORT_MAX = 4096;
+
+// 16-bit order key of an fp16 score (NaN lowest, as key_f32)
+__device__ inline uint32_t key_h16(uint16_t h) {
+    if ((h & 0x7c00u) == 0x7c00u && (h & 0x03ffu)) return 0u;
+    return (h & 0x8000u) ? (~h & 0xffffu) : (h | 0x8000u);
+}
+
+__device__ inline uint32_t seg_key(const void* s, int dtype, int64_t i) {
+    if (dtype == SSEQS_F16) return key_h16(static_cast<const uint16_t*>(s)[i]);
+    return load_key(s, dtype, i);
+}
+
+__global__ __launch_bounds__(SEG_THREADS) void k_seg_topk(
+    const void* __restrict__ s, int dtype, const int64_t* __restrict__ seg_begin,
+    const int32_t* __restrict__ seg_n, const int32_t* __restrict__ seg_k,
+    const int64_t* __restrict__ out_base, int64_t nseg, int sorted, int drop_front,
+    int drop_back, int add_begin, int64_t* __restrict__ out) {
+    __shared__ uint32_t hist[SEL_BINS];
+    __shared__ uint64_t comp[SEG_SORT_MAX];
+    __shared__ int wg[SEG_WAVES], we[SEG_WAVES];
+    __shared__ uint32_t s_pick[2];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int npass = dtype == SSEQS_F16 ? 2 : 3;
+    const int shifts16[2] = {5, 0}, bits16[2] = {11, 5};
+    const int shifts32[3] = {21, 10, 0}, bits32[3] = {11, 11, 10};
+    for (int64_t j = blockIdx.x; j < nseg; j += gridDim.x) {
+        const int64_t beg = seg_begin[j];
+        const int n = seg_n[j];
+        int k = min(seg_k[j], n);
+        if (sorted) k = min(k, SEG_SORT_MAX);
+        if (k <= 0) continue;  // uniform over the block
+        const void* base = dtype == SSEQS_F16
+                               ? static_cast<const void*>(static_cast<const uint16_t*>(s) + beg)
+                           : dtype == SSEQS_F32
+                               ? static_cast<const void*>(static_cast<const float*>(s) + beg)
+                               : static_cast<const void*>(static_cast<const int32_t*>(s) + beg);
+        uint32_t prefix = 0, mask = 0, krem = (uint32_t)k;
+        for (int p = 0; p < npass; ++p) {
+            const int shift = dtype == SSEQS_F16 ? shifts16[p] : shifts32[p];
+            const int bits = dtype == SSEQS_F16 ? bits16[p] : bits32[p];
+            const uint32_t bm = (1u << bits) - 1u;
+            for (int i = tid; i < SEL_BINS; i += SEG_THREADS) hist[i] = 0;
+            __syncthreads();
+            for (int i = tid; i < n; i += SEG_THREADS) {
+                const uint32_t key = seg_key(base, dtype, i);
+                if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & bm], 1u);
+            }
+            __syncthreads();
+            if (wid == 0) {
+                // lane L holds bins 32L..32L+31; the lane whose bins contain
+                // the krem-th largest key walks them from the top
+                uint32_t sum = 0;
+                for (int t = 0; t < 32; ++t) sum += hist[lane * 32 + t];
+                uint32_t incl = sum;  // keys in bins of lanes >= this one
+                for (int off = 1; off < 64; off <<= 1) {
+                    const uint32_t t = __shfl_down(incl, off, 64);
+                    if (lane + off < 64) incl += t;
+                }
+                uint32_t above = incl - sum;
+                if (above < krem && krem <= incl) {
+                    int b = lane * 32 + 31;
+                    for (; b > lane * 32; --b) {
+                        if (above + hist[b] >= krem) break;
+                        above += hist[b];
+                    }
+                    s_pick[0] = (uint32_t)b;
+                    s_pick[1] = krem - above;
+                }
+            }
+            __syncthreads();
+            prefix |= s_pick[0] << shift;
+            mask |= bm << shift;
+            krem = s_pick[1];
+            __syncthreads();
+        }
+        // keys > T are all taken, keys == T while fewer than krem came before
+        const uint32_t T = prefix;
+        const int keq = (int)krem;
+        const int hi = k - drop_back;
+        const int64_t ob = out_base[j] - drop_front;
+        const int64_t add = add_begin ? beg : 0;
+        int gt_run = 0, eq_run = 0;
+        for (int i0 = 0; i0 < n; i0 += SEG_THREADS) {
+            const int i = i0 + tid;
+            const uint32_t key = i < n ? seg_key(base, dtype, i) : 0u;
+            const bool g = i < n && key > T;
+            const bool e = i < n && key == T;
+            const uint64_t mg = __ballot(g), me = __ballot(e);
+            const uint64_t below = (1ull << lane) - 1ull;
+            if (lane == 0) {
+                wg[wid] = __popcll(mg);
+                we[wid] = __popcll(me);
+            }
+            __syncthreads();
+            int og = 0, oe = 0, tg = 0, te = 0;
+            for (int w = 0; w < SEG_WAVES; ++w) {
+                const int a = wg[w], b = we[w];
+                if (w < wid) {
+                    og += a;
+                    oe += b;
+                }
+                tg += a;
+                te += b;
+            }
+            const int gb = gt_run + og + __popcll(mg & below);
+            const int eb = eq_run + oe + __popcll(me & below);
+            const int slot = gb + min(eb, keq);
+            if ((g || (e && eb < keq)) && slot < k) {
+                if (sorted)
+                    comp[slot] = ((uint64_t)(~key) << 32) | (uint64_t)(uint32_t)i;
+                else if (slot >= drop_front && slot < hi)
+                    out[ob + slot] = add + i;
+            }
+            gt_run += tg;
+            eq_run += te;
+            __syncthreads();
+        }
+        if (sorted) {
+            int P = 1;
+            while (P < k) P <<= 1;
+            for (int i = k + tid; i < P; i += SEG_THREADS) comp[i] = ~0ull;
+            __syncthreads();
+            for (int size = 2; size <= P; size <<= 1) {
+                for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                    for (int i = tid; i < P; i += SEG_THREADS) {
+                        const int m = i ^ stride;
+                        if (m > i) {
+                            const bool up = (i & size) == 0;
+                            const uint64_t a = comp[i], b = comp[m];
+                            if ((a > b) == up) {
+                                comp[i] = b;
+                                comp[m] = a;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            }
+            for (int i = drop_front + tid; i < hi; i += SEG_THREADS)
+                out[ob + i] = add + (int64_t)(uint32_t)comp[i];
+            __syncthreads();
+        }
+    }
+}
+
 }  // namespace sseqs
 
 using namespace sseqs;
@@ -282,4 +438,25 @@ extern "C" int sseqs_topk(const void* scores, int dtype, int64_t n, int64_t k, i
         if (int r = check_launch("topk sort")) return r;
     }
     return 0;
+}
+
+extern "C" int sseqs_topk_segments(const void* scores, int dtype, const int64_t* seg_begin,
+                                   const int32_t* seg_n, const int32_t* seg_k,
+                                   const int64_t* out_base, int64_t nseg, int max_k, int sorted,
+                                   int drop_front, int drop_back, int add_begin,
+                                   int64_t* out_idx, hipStream_t stream) {
+    if (nseg < 0 || max_k < 0 || drop_front < 0 || drop_back < 0)
+        return fail(-1, "topk_segments: negative size");
+    if (dtype != SSEQS_F32 && dtype != SSEQS_F16 && dtype != SSEQS_I32)
+        return fail(-1, "topk_segments: unsupported dtype");
+    if (sorted && max_k > SEG_SORT_MAX)
+        return fail(-1, "topk_segments: sorted output limited to k <= 4096 per segment");
+    if (nseg == 0 || max_k == 0) return 0;
+    if (!scores || !seg_begin || !seg_n || !seg_k || !out_base || !out_idx)
+        return fail(-1, "topk_segments: null pointer");
+    const unsigned grid = (unsigned)(nseg < 8192 ? nseg : 8192);
+    hipLaunchKernelGGL(k_seg_topk, dim3(grid), dim3(SEG_THREADS), 0, stream, scores, dtype,
+                       seg_begin, seg_n, seg_k, out_base, nseg, sorted, drop_front, drop_back,
+                       add_begin, out_idx);
+    return check_launch("topk_segments");
 }

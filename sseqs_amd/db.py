@@ -106,6 +106,15 @@ class SeqDB:
         self.extra[key] = bounds
         return bounds
 
+    def batch_plan(self, bs: int):
+        """Segments of every batch of ``bs`` packed bytes (cached per bs);
+        see sseqs_amd.batches."""
+        key = ("batch_plan", int(bs))
+        if key not in self.extra:
+            from .batches import plan_batches
+            self.extra[key] = plan_batches(self.seq_off, self.seq_len, self.n_syms, int(bs))
+        return self.extra[key]
+
     # ----------------------------------------------------------- gathers --
     def gather(self, ids: torch.Tensor, width: int = 500) -> tuple[torch.Tensor, torch.Tensor]:
         """ASCII rows (n, width) and lengths of sequences ``ids`` (index order),

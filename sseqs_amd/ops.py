@@ -17,6 +17,7 @@ from __future__ import annotations
 
 from types import SimpleNamespace
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -190,3 +191,47 @@ def topk_indices(scores: torch.Tensor, k: int, sorted: bool = True) -> torch.Ten
     call("sseqs_topk", ptr(s), _DT[s.dtype], n, k, int(bool(sorted)), ptr(out), ptr(ws),
          ws.numel(), stream_of(s.device))
     return out
+
+
+def topk_segments(scores: torch.Tensor, begin, n, k, *, sorted: bool = False,
+                  drop_front: int = 0, drop_back: int = 0, add_begin: bool = False
+                  ) -> tuple[torch.Tensor, np.ndarray]:
+    """topk_indices of many segments of ``scores`` in one launch.
+
+    Segment j is ``scores[begin[j] : begin[j] + n[j]]`` and selects its
+    ``k[j]`` largest (host integer arrays).  Of each selection, in
+    ascending index order (or (score desc, index asc) when ``sorted``,
+    k <= 4096), entries ``drop_front .. k[j] - drop_back - 1`` are kept.
+    Returns the kept indices of all segments concatenated (int64, within
+    the segment, or within ``scores`` when ``add_begin``) and the start of
+    each segment's part.
+    """
+    s = scores.contiguous().view(-1)
+    if s.dtype not in _DT:
+        raise TypeError(f"topk_segments: unsupported dtype {s.dtype}")
+    begin = np.asarray(begin, dtype=np.int64)
+    n = np.asarray(n, dtype=np.int64)
+    k = np.minimum(np.asarray(k, dtype=np.int64), n)
+    if begin.shape != n.shape or begin.shape != k.shape:
+        raise ValueError("topk_segments: begin, n and k must have one entry per segment")
+    if np.any(n < 0) or np.any(k < 0) or drop_front < 0 or drop_back < 0:
+        raise ValueError("topk_segments: negative size")
+    if n.size and (np.any(begin < 0) or np.any(begin + n > s.numel())):
+        raise ValueError("topk_segments: segment outside the scores")
+    if sorted and k.size and int(k.max()) > 4096:
+        raise ValueError("topk_segments: sorted output limited to k <= 4096 per segment")
+    cnt = np.maximum(k - drop_front - drop_back, 0)
+    base = np.zeros(cnt.size, dtype=np.int64)
+    if cnt.size > 1:
+        base[1:] = np.cumsum(cnt)[:-1]
+    total = int(cnt.sum())
+    out = torch.empty(total, dtype=torch.int64, device=s.device)
+    if total == 0:
+        return out, base
+    meta64 = torch.from_numpy(np.concatenate([begin, base])).to(s.device)
+    meta32 = torch.from_numpy(np.concatenate([n, k]).astype(np.int32)).to(s.device)
+    m = begin.size
+    call("sseqs_topk_segments", ptr(s), _DT[s.dtype], ptr(meta64), ptr(meta32),
+         ptr(meta32[m:]), ptr(meta64[m:]), m, int(k.max()), int(bool(sorted)), int(drop_front),
+         int(drop_back), int(bool(add_begin)), ptr(out), stream_of(s.device))
+    return out, base

@@ -214,90 +214,248 @@ def batch_search(db: SeqDB, query: str, q: PreparedQuery, bs: int = 50_000_000,
                  top_fft: int = 200, top_sw: int = 10, top_sw_affine: int = 2,
                  dlength: float = 1e6, chunk_bytes: int = 1_000_000_000,
                  timings: dict | None = None, progress=None,
-                 hit_width: int | None = 500) -> list[BatchHit]:
-    """The reference cascade of sseqs/msa.py:196-370 on the resident stream.
+                 hit_width: int | None = 500, on_scores=None,
+                 resident: bool = True) -> list[BatchHit]:
+    """The reference cascade of sseqs/msa.py:196-370 over the resident stream.
+
     Hits keep their first ``hit_width`` residues (500 in the reference,
-    sseqs/msa.py:183,367; None keeps them whole)."""
-    dev = db.device
-    hits: list[BatchHit] = []
-    Lq = len(query)
-    nbytes = db.nbytes
+    sseqs/msa.py:183,367; None keeps them whole).  ``on_scores(b, scores)``,
+    if given, receives the fp16 prefilter scores of every batch b that
+    reaches the prefilter.
+
+    With ``resident`` (and bs a multiple of 5, so batches start on whole
+    8-symbol groups) the segments of every batch come from the resident
+    index and all batches run together: one prefilter launch per run of
+    batches with the same correlation size, then segmented selects and SW
+    launches over all batches at once, and one read-back of the hits.
+    Batches whose reference offsets are shifted by empty segments
+    (sseqs_amd.batches) and every batch of other configurations run the
+    per-batch ASCII path of the reference.
+    """
     t = timings if timings is not None else {}
-    scratch = torch.zeros(1, dtype=torch.float16, device=dev)
-    for lo in range(0, nbytes, bs):
-        chunk, j = lo // chunk_bytes, (lo % chunk_bytes) // bs
-        nb = min(bs, nbytes - lo)
-        n_sym = min(nb * 8 // 5, db.n_syms - (lo * 8) // 5)
+    Lq_sw = int(q.codes.numel())
+    if (resident and bs > 0 and bs % 5 == 0 and 1 <= Lq_sw <= 2048 and top_fft > 0
+            and top_sw > 0 and 1 <= top_sw_affine <= 4096):
+        t0 = time.time()
+        plan = db.batch_plan(bs)
+        t["t_plan"] = t.get("t_plan", 0.0) + time.time() - t0
+        return _resident_batches(db, query, q, plan, top_fft, top_sw, top_sw_affine, dlength,
+                                 chunk_bytes, t, progress, hit_width, on_scores)
+    hits: list[BatchHit] = []
+    for lo in range(0, db.nbytes, bs):
+        n_sym = min(min(bs, db.nbytes - lo) * 8 // 5, db.n_syms - (lo * 8) // 5)
         if n_sym <= 0:
             continue
-        t0 = time.time()
-        ascii = ops.decompress(db.packed, n_sym, lo)
-        t["t_dec"] = t.get("t_dec", 0.0) + time.time() - t0
-        t0 = time.time()
-        D = torch.nonzero(ascii == 64).flatten()                       # msa.py:227
-        starts = D + 1
-        starts = starts[starts < ascii.numel()]                        # msa.py:238
-        offsets = torch.cat([starts, torch.tensor([ascii.numel()], device=dev)])
-        lengths = offsets - torch.cat([torch.zeros(1, dtype=offsets.dtype, device=dev),
-                                       offsets[:-1]])
-        lengths[:-1] -= 1
-        if int(ascii[-1].item()) == 64:
-            lengths[-1] -= 1
-        max_seq_length = float(lengths.max().item())
-        t["t_prep1"] = t.get("t_prep1", 0.0) + time.time() - t0
-        if progress is not None:
-            progress(lo, nb, max_seq_length)
-        if abs(Lq - max_seq_length) > dlength:                         # msa.py:252
-            continue
-        lengths = lengths[lengths != 0]
-        if lengths.numel() == 0:
-            continue
-        t0 = time.time()
-        maxlen = int(lengths.max().item())
-        n = _nextpow2(max(Lq, maxlen))
-        lengths32 = lengths.to(torch.int32)
-        fft_off = torch.zeros(lengths.numel() + 1, dtype=torch.int64, device=dev)
-        fft_off[1:] = torch.cumsum(lengths + 1, 0)
-        scores = torch.zeros(lengths.numel(), dtype=torch.float16, device=dev)
-        ops.warp_corr(scores, q.qrev, q.lut_a, q.lut_b,
-                      ascii, fft_off[:-1].to(torch.int32), lengths32, q.k, Lq, maxlen, n,
-                      scratch)
-        t["t_fft"] = t.get("t_fft", 0.0) + time.time() - t0
-        t0 = time.time()
-        good_idx = ops.topk_indices(scores, scores.numel() // top_fft, sorted=False)
-        good_idx = good_idx[1:-1]                                       # msa.py:307
-        if good_idx.numel() == 0:
-            continue
-        gi32 = good_idx.to(torch.int32)
-        sw_scores = ops.sw_cuda_blosum62(q.codes, gi32, ascii, D.to(torch.int32),
-                                         gap_penalty_param=-1)
-        ktop = sw_scores.numel() // top_sw
-        if ktop < top_sw_affine:   # the reference raises inside topk here
-            continue
-        _topk = ops.topk_indices(sw_scores, ktop, sorted=True)
-        gap = ops.sw_cuda_affine(q.codes, gi32[_topk], ascii, starts.to(torch.int32), 11, 1)
-        __topk = ops.topk_indices(gap, top_sw_affine, sorted=True)
-        gap_sel = gap[__topk]
-        pos = _topk[__topk]
-        norm = gap_sel.float() / torch.tensor(float(Lq * maxlen), device=dev).sqrt()
-        seg = good_idx[pos]
-        hs, hl = fft_off[:-1][seg], lengths[seg]
-        width = int(hl.max().item())
-        if hit_width is not None:
-            width = min(int(hit_width), width)
-        idx = hs[:, None] + torch.arange(width, device=dev)[None, :]
-        rows = ascii[idx.clamp_max(ascii.numel() - 1)]
-        t["t_sw"] = t.get("t_sw", 0.0) + time.time() - t0
-        rows_h = rows.cpu().numpy()
-        hl_h = hl.cpu().numpy()
-        norm_h = norm.cpu().numpy()
-        gap_h = gap_sel.cpu().numpy()
-        lin_h = sw_scores[pos].cpu().numpy()
-        fft_h = scores[pos].float().cpu().numpy()
-        for r in range(rows_h.shape[0]):
-            L = min(int(hl_h[r]), width)
-            hits.append(BatchHit(chunk, j, float(norm_h[r]), int(gap_h[r]), int(lin_h[r]),
-                                 float(fft_h[r]), rows_h[r, :L].tobytes().decode("latin-1")))
+        hits += _ascii_batch(db, query, q, lo, bs, top_fft, top_sw, top_sw_affine, dlength,
+                             chunk_bytes, t, progress, hit_width, on_scores)
+    return hits
+
+
+def _ascii_batch(db: SeqDB, query: str, q: PreparedQuery, lo: int, bs: int, top_fft: int,
+                 top_sw: int, top_sw_affine: int, dlength: float, chunk_bytes: int, t: dict,
+                 progress, hit_width: int | None, on_scores) -> list[BatchHit]:
+    """One batch of the reference cascade from its ASCII unpacking
+    (sseqs/msa.py:211-370)."""
+    dev = db.device
+    Lq = len(query)
+    chunk, j = lo // chunk_bytes, (lo % chunk_bytes) // bs
+    nb = min(bs, db.nbytes - lo)
+    n_sym = min(nb * 8 // 5, db.n_syms - (lo * 8) // 5)
+    t0 = time.time()
+    ascii = ops.decompress(db.packed, n_sym, lo)
+    t["t_dec"] = t.get("t_dec", 0.0) + time.time() - t0
+    t0 = time.time()
+    D = torch.nonzero(ascii == 64).flatten()                       # msa.py:227
+    starts = D + 1
+    starts = starts[starts < ascii.numel()]                        # msa.py:238
+    offsets = torch.cat([starts, torch.tensor([ascii.numel()], device=dev)])
+    lengths = offsets - torch.cat([torch.zeros(1, dtype=offsets.dtype, device=dev),
+                                   offsets[:-1]])
+    lengths[:-1] -= 1
+    if int(ascii[-1].item()) == 64:
+        lengths[-1] -= 1
+    max_seq_length = float(lengths.max().item())
+    t["t_prep1"] = t.get("t_prep1", 0.0) + time.time() - t0
+    if progress is not None:
+        progress(lo, nb, max_seq_length)
+    if abs(Lq - max_seq_length) > dlength:                         # msa.py:252
+        return []
+    lengths = lengths[lengths != 0]
+    if lengths.numel() == 0:
+        return []
+    t0 = time.time()
+    maxlen = int(lengths.max().item())
+    n = _nextpow2(max(Lq, maxlen))
+    lengths32 = lengths.to(torch.int32)
+    fft_off = torch.zeros(lengths.numel() + 1, dtype=torch.int64, device=dev)
+    fft_off[1:] = torch.cumsum(lengths + 1, 0)
+    scores = torch.zeros(lengths.numel(), dtype=torch.float16, device=dev)
+    ops.warp_corr(scores, q.qrev, q.lut_a, q.lut_b,
+                  ascii, fft_off[:-1].to(torch.int32), lengths32, q.k, Lq, maxlen, n,
+                  torch.zeros(1, dtype=torch.float16, device=dev))
+    if on_scores is not None:
+        on_scores(lo // bs, scores)
+    t["t_fft"] = t.get("t_fft", 0.0) + time.time() - t0
+    t0 = time.time()
+    good_idx = ops.topk_indices(scores, scores.numel() // top_fft, sorted=False)
+    good_idx = good_idx[1:-1]                                       # msa.py:307
+    if good_idx.numel() == 0:
+        return []
+    gi32 = good_idx.to(torch.int32)
+    sw_scores = ops.sw_cuda_blosum62(q.codes, gi32, ascii, D.to(torch.int32),
+                                     gap_penalty_param=-1)
+    ktop = sw_scores.numel() // top_sw
+    if ktop < top_sw_affine:   # the reference raises inside topk here
+        return []
+    _topk = ops.topk_indices(sw_scores, ktop, sorted=True)
+    gap = ops.sw_cuda_affine(q.codes, gi32[_topk], ascii, starts.to(torch.int32), 11, 1)
+    __topk = ops.topk_indices(gap, top_sw_affine, sorted=True)
+    gap_sel = gap[__topk]
+    pos = _topk[__topk]
+    norm = gap_sel.float() / torch.tensor(float(Lq * maxlen), device=dev).sqrt()
+    seg = good_idx[pos]
+    hs, hl = fft_off[:-1][seg], lengths[seg]
+    width = int(hl.max().item())
+    if hit_width is not None:
+        width = min(int(hit_width), width)
+    idx = hs[:, None] + torch.arange(width, device=dev)[None, :]
+    rows = ascii[idx.clamp_max(ascii.numel() - 1)]
+    t["t_sw"] = t.get("t_sw", 0.0) + time.time() - t0
+    rows_h = rows.cpu().numpy()
+    hl_h = hl.cpu().numpy()
+    norm_h = norm.cpu().numpy()
+    gap_h = gap_sel.cpu().numpy()
+    lin_h = sw_scores[pos].cpu().numpy()
+    fft_h = scores[pos].float().cpu().numpy()
+    hits = []
+    for r in range(rows_h.shape[0]):
+        L = min(int(hl_h[r]), width)
+        hits.append(BatchHit(chunk, j, float(norm_h[r]), int(gap_h[r]), int(lin_h[r]),
+                             float(fft_h[r]), rows_h[r, :L].tobytes().decode("latin-1")))
+    return hits
+
+
+def _resident_batches(db: SeqDB, query: str, q: PreparedQuery, plan, top_fft: int,
+                      top_sw: int, top_sw_affine: int, dlength: float, chunk_bytes: int,
+                      t: dict, progress, hit_width: int | None, on_scores) -> list[BatchHit]:
+    """All batches of one sweep at once from the batch plan; the same
+    cascade as _ascii_batch, batch by batch, on the plan's segments."""
+    dev = db.device
+    st = stream_of(dev)
+    Lq = len(query)
+    Lq_sw = int(q.codes.numel())
+    bs = plan.bs
+    maxl = plan.maxlen
+    if progress is not None:
+        for b in range(plan.n_batches):
+            progress(int(plan.lo[b]), int(min(bs, db.nbytes - plan.lo[b])), float(maxl[b]))
+    active = (plan.nseg > 0) & ~(np.abs(Lq - maxl.astype(np.float64)) > dlength)
+    fast = active & plan.regular
+    slow = np.nonzero(active & ~plan.regular)[0]
+    Ns = np.array([_nextpow2(max(Lq, int(m))) for m in maxl], dtype=np.int64)
+    bad = fast & ((Ns < 16) | (Ns > 2048))
+    if bad.any():
+        raise ValueError(f"warp_corr: n must be a power of two in 16..2048, got "
+                         f"{int(Ns[np.nonzero(bad)[0][0]])}")
+    fb = np.nonzero(fast)[0]
+
+    # prefilter: one launch per run of consecutive batches with one N
+    t0 = time.time()
+    sc32 = torch.empty(plan.seg_off.numel(), dtype=torch.float32, device=dev)
+    i = 0
+    while i < fb.size:
+        e = i
+        while e + 1 < fb.size and fb[e + 1] == fb[e] + 1 and Ns[fb[e + 1]] == Ns[fb[i]]:
+            e += 1
+        b0, b1 = fb[i], fb[e]
+        s_begin = int(plan.seg_base[b0])
+        s_end = int(plan.seg_base[b1] + plan.nseg[b1])
+        call("sseqs_db_prefilter", ptr(db.packed), ptr(plan.seg_off), ptr(plan.seg_len), s_begin,
+             s_end, ptr(q.qrev), Lq, ptr(q.lut_a), ptr(q.lut_b_flat), 21, q.k, int(Ns[b0]),
+             int(maxl[b0:b1 + 1].max()), sc32.data_ptr() + 4 * s_begin, st)
+        i = e + 1
+    sc16 = sc32.half()
+    del sc32
+    if on_scores is not None:
+        for b in fb:
+            on_scores(int(b), sc16[plan.seg_base[b]: plan.seg_base[b] + plan.nseg[b]])
+    t["t_fft"] = t.get("t_fft", 0.0) + time.time() - t0
+
+    # top 1/top_fft of each batch in index order, without its first and
+    # last (sseqs/msa.py:297,307), as segment ids; linear SW -1 of those
+    t0 = time.time()
+    k1 = plan.nseg[fb] // top_fft
+    cand1, base1 = ops.topk_segments(sc16, plan.seg_base[fb], plan.nseg[fb], k1, drop_front=1,
+                                     drop_back=1, add_begin=True)
+    cnt1 = np.maximum(k1 - 2, 0)
+    sw1 = torch.empty(cand1.numel(), dtype=torch.int32, device=dev)
+    if cand1.numel():
+        call("sseqs_db_sw_linear", ptr(db.packed), ptr(plan.seg_off), ptr(plan.seg_len),
+             ptr(cand1), cand1.numel(), ptr(q.codes), Lq_sw, -1, ptr(sw1), st)
+    # top 1/top_sw by linear score (sorted), affine SW 11/1 of segment[1:]+'@'
+    k2 = cnt1 // top_sw
+    ok = (cnt1 > 0) & (k2 >= top_sw_affine)
+    b2, base1o, k2o = fb[ok], base1[ok], k2[ok]
+    if k2o.size and int(k2o.max()) > 4096:
+        parts = [ops.topk_indices(sw1[int(o): int(o) + int(c)], int(k), sorted=True)
+                 for o, c, k in zip(base1o, cnt1[ok], k2o)]
+        pos2 = torch.cat(parts) if parts else torch.empty(0, dtype=torch.int64, device=dev)
+        base2 = np.zeros(k2o.size, dtype=np.int64)
+        base2[1:] = np.cumsum(k2o)[:-1]
+    else:
+        pos2, base2 = ops.topk_segments(sw1, base1o, cnt1[ok], k2o, sorted=True)
+    rep = torch.from_numpy(np.repeat(base1o, k2o)).to(dev)
+    ids2 = cand1[rep + pos2]
+    n2 = int(ids2.numel())
+    sw2 = torch.empty(n2, dtype=torch.int32, device=dev)
+    if n2:
+        aff_off = (plan.seg_off[ids2] + 1).contiguous()
+        aff_len = plan.seg_len[ids2].contiguous()
+        call("sseqs_db_sw_affine", ptr(db.packed), ptr(aff_off), ptr(aff_len),
+             ptr(torch.arange(n2, device=dev)), n2, ptr(q.codes), Lq_sw, 11, 1, ptr(sw2), st)
+    # top top_sw_affine by affine score (sorted); the hits' fields
+    k3 = np.full(b2.size, top_sw_affine, dtype=np.int64)
+    pos3, _ = ops.topk_segments(sw2, base2, k2o, k3, sorted=True)
+    g3 = torch.from_numpy(np.repeat(base2, k3)).to(dev) + pos3
+    p2 = pos2[g3]                                    # index into the batch's stage-1 list
+    at1 = torch.from_numpy(np.repeat(base1o, k3)).to(dev) + p2
+    gseg = cand1[at1]
+    gap = sw2[g3]
+    lin = sw1[at1]
+    fft = sc16[torch.from_numpy(np.repeat(plan.seg_base[b2], k3)).to(dev) + p2].float()  # (sic)
+    den = torch.from_numpy(np.repeat(Lq * maxl[b2], k3).astype(np.float64)).to(
+        device=dev, dtype=torch.float32).sqrt()
+    norm = gap.float() / den
+    width = int(hit_width) if hit_width is not None else (
+        int(plan.seg_len[gseg].max().item()) if gseg.numel() else 1)
+    rows = torch.zeros((gseg.numel(), max(width, 1)), dtype=torch.uint8, device=dev)
+    hl = torch.empty(gseg.numel(), dtype=torch.int32, device=dev)
+    if gseg.numel():
+        call("sseqs_db_gather", ptr(db.packed), ptr(plan.seg_off), ptr(plan.seg_len), ptr(gseg),
+             gseg.numel(), max(width, 1), ptr(rows), ptr(hl), st)
+    rows_h, hl_h = rows.cpu().numpy(), hl.cpu().numpy()
+    norm_h, gap_h = norm.cpu().numpy(), gap.cpu().numpy()
+    lin_h, fft_h = lin.cpu().numpy(), fft.cpu().numpy()
+    t["t_sw"] = t.get("t_sw", 0.0) + time.time() - t0
+
+    per: dict[int, list[BatchHit]] = {}
+    r = 0
+    for b in b2:
+        lo = int(plan.lo[b])
+        chunk, j = lo // chunk_bytes, (lo % chunk_bytes) // bs
+        out = []
+        for _ in range(top_sw_affine):
+            out.append(BatchHit(chunk, j, float(norm_h[r]), int(gap_h[r]), int(lin_h[r]),
+                                float(fft_h[r]), rows_h[r, : hl_h[r]].tobytes().decode("latin-1")))
+            r += 1
+        per[int(b)] = out
+    for b in slow:
+        per[int(b)] = _ascii_batch(db, query, q, int(plan.lo[b]), bs, top_fft, top_sw,
+                                   top_sw_affine, dlength, chunk_bytes, t, None, hit_width,
+                                   on_scores)
+    hits: list[BatchHit] = []
+    for b in sorted(per):
+        hits += per[b]
     return hits
 
 

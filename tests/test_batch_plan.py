@@ -1,0 +1,83 @@
+"""The batch plan (segments of every batch from the resident index) against
+the oracle's segmentation of each batch's ASCII window (no GPU needed)."""
+import numpy as np
+import pytest
+import torch
+
+from sseqs_amd.batches import plan_batches
+
+AA = np.frombuffer(b"ARNDCQEGHILKMFPSTWYVXBZ", dtype=np.uint8)
+
+
+def make_stream(rng, n_runs, lead_at, gaps_of_two):
+    """ASCII stream of runs separated by '@' (sometimes '@@' or more)."""
+    parts = [b"@" * lead_at]
+    for i in range(n_runs):
+        L = int(rng.integers(1, 60))
+        parts.append(rng.choice(AA, L).tobytes())
+        gap = 1
+        if gaps_of_two and rng.random() < 0.1:
+            gap = int(rng.integers(2, 5))
+        parts.append(b"@" * gap)
+    return np.frombuffer(b"".join(parts), dtype=np.uint8)
+
+
+def runs_of(a):
+    """The resident index: every maximal run of non-'@' symbols."""
+    x = np.concatenate([[64], a, [64]])
+    nz = (x != 64).astype(np.int8)
+    d = np.diff(nz)
+    st = np.nonzero(d == 1)[0]
+    en = np.nonzero(d == -1)[0]
+    return st.astype(np.int64), (en - st).astype(np.int64)
+
+
+@pytest.mark.parametrize("seed,lead_at,gaps,bs,shuffle", [
+    (0, 0, False, 5, False), (1, 0, False, 25, False), (2, 1, False, 10, True),
+    (3, 0, True, 15, True), (4, 2, True, 100, False), (5, 0, True, 5, True),
+    (6, 0, False, 1000, False),
+])
+def test_plan_matches_oracle_segments(orc, seed, lead_at, gaps, bs, shuffle):
+    rng = np.random.default_rng(seed)
+    a = make_stream(rng, 300, lead_at, gaps)
+    off, ln = runs_of(a)
+    if shuffle:  # the index may be in length order rather than stream order
+        p = rng.permutation(off.size)
+        off, ln = off[p], ln[p]
+    plan = plan_batches(torch.from_numpy(off), torch.from_numpy(ln.astype(np.int32)), a.size, bs)
+    sb = bs * 8 // 5
+    nbytes = (a.size * 5 + 7) // 8
+    assert plan.n_batches == -(-nbytes // bs)
+    seg_off = plan.seg_off.numpy()
+    seg_len = plan.seg_len.numpy()
+    n_regular = n_irregular = 0
+    for b in range(plan.n_batches):
+        s0 = b * sb
+        w = a[s0: s0 + plan.n_sym[b]]
+        assert plan.n_sym[b] == min(sb, a.size - s0)
+        D, starts, lengths, fft_off = orc.segments(w)
+        assert plan.nseg[b] == lengths.size
+        assert plan.maxlen[b] == (int(lengths.max()) if lengths.size else 0)
+        # true start of every non-empty segment of the window
+        x = np.concatenate([[-1], D, [w.size]])
+        true_st = np.array([x[i] + 1 for i in range(x.size - 1) if x[i + 1] - x[i] > 1],
+                           dtype=np.int64)
+        regular = lengths.size > 0 and np.array_equal(fft_off[:-1], true_st)
+        assert bool(plan.regular[b]) == regular
+        if regular:
+            n_regular += 1
+            sl = slice(plan.seg_base[b], plan.seg_base[b] + plan.nseg[b])
+            assert np.array_equal(seg_off[sl] - s0, fft_off[:-1])
+            assert np.array_equal(seg_len[sl], lengths)
+        elif lengths.size:
+            n_irregular += 1
+    assert n_regular > 0
+    if lead_at or gaps:
+        assert n_irregular > 0
+
+
+def test_plan_rejects_unaligned_batches():
+    off = torch.tensor([0], dtype=torch.int64)
+    ln = torch.tensor([3], dtype=torch.int32)
+    with pytest.raises(ValueError):
+        plan_batches(off, ln, 3, 7)

@@ -312,6 +312,48 @@ def test_topk_errors(dev):
     assert ops.topk_indices(t, 0).numel() == 0
 
 
+@pytest.mark.parametrize("dtype", [torch.float16, torch.int32, torch.float32])
+def test_topk_segments(orc, dev, dtype):
+    """Per-segment selects in one launch equal the oracle's top-k of each
+    segment (ties to the smaller index), in both orders, with the first
+    and last entries dropped and with segment-relative or absolute ids."""
+    from sseqs_amd import ops
+    announce(f"topk_segments {dtype}")
+    rng = np.random.default_rng(7)
+    n = np.array([0, 1, 5, 300, 4097, 70000, 1000, 250000, 33], dtype=np.int64)
+    begin = np.concatenate([[0], np.cumsum(n)[:-1]]) + 3
+    total = int(begin[-1] + n[-1]) + 5
+    if dtype == torch.int32:
+        x = rng.integers(-40, 40, total).astype(np.int32)
+    else:
+        x = np.round(rng.standard_normal(total), 2).astype(np.float32)
+        x[x == 0] = 0.0  # no -0.0: the keys order it below +0.0, the oracle ties them
+        x[rng.integers(0, total, total // 100)] = -np.inf
+        if dtype == torch.float16:
+            x = x.astype(np.float16)
+    t = torch.from_numpy(x).to(dev)
+    for sorted_, kfun in ((False, lambda m: m // 7), (True, lambda m: min(m, 4096) // 3)):
+        k = np.array([kfun(int(m)) for m in n], dtype=np.int64)
+        k[2] = 5  # k == n
+        for drop, add in (((0, 0), False), ((1, 1), True)):
+            got, base = ops.topk_segments(t, begin, n, k, sorted=sorted_, drop_front=drop[0],
+                                          drop_back=drop[1], add_begin=add)
+            got = got.cpu().numpy()
+            for j in range(n.size):
+                seg = x[begin[j]: begin[j] + n[j]].astype(np.float64)
+                want = orc.topk_desc(seg, int(k[j]))
+                if not sorted_:
+                    want = np.sort(want)
+                want = want[drop[0]: max(int(k[j]) - drop[1], drop[0])]
+                if add:
+                    want = want + begin[j]
+                part = got[base[j]: base[j] + want.size]
+                assert np.array_equal(part, want), (j, sorted_, drop)
+            assert got.size == sum(max(int(kk) - drop[0] - drop[1], 0) for kk in k)
+    with pytest.raises(ValueError):
+        ops.topk_segments(t, begin[:1], n[7:8], [5000], sorted=True)
+
+
 # ------------------------------------------------------------- backtrack --
 
 def test_backtrack_matches_oracle(orc, dev):

@@ -94,42 +94,82 @@ def test_global_cascade_linear_gap(orc, world):
     assert np.array_equal(gl.ids.cpu().numpy(), cand[top])
 
 
-def test_batch_cascade(orc, world):
+def mutate(rng, s, sub=0.3, indel=0.03):
+    aa = "ARNDCQEGHILKMFPSTWYV"
+    out = []
+    for c in s:
+        r = rng.random()
+        if r < indel / 2:
+            continue
+        if r < indel:
+            out.append("".join(rng.choice(list(aa), int(rng.integers(1, 4)))))
+        out.append(str(rng.choice(list(aa))) if rng.random() < sub else c)
+    return "".join(out) or "A"
+
+
+def _check_batch_cascade(orc, db, ascii, query, bs, resident, **kw):
+    """batch_search against the oracle cascade fed with the fp16 prefilter
+    scores batch_search itself used (checked against FP64 first)."""
     from sseqs_amd.query import prepare
     from sseqs_amd.search import batch_search
-    announce("batch cascade")
-    db, query = world["db"], world["queries"][0]
     q = prepare(query)
-    bs = 1_000_000  # bytes -> 1.6 M symbols per batch, several batches
-    got = batch_search(db, query, q, bs=bs, top_fft=50, top_sw=5, top_sw_affine=2)
-    # feed the oracle the device's own fp16 prefilter scores batch by batch
-    from sseqs_amd import ops
-    dev_scores = []
-    from sseqs_amd.search import _nextpow2
-
-    def fft_scores(j):
-        return dev_scores[j]
-    for lo in range(0, db.nbytes, bs):
-        n_sym = min(bs * 8 // 5, db.n_syms - lo * 8 // 5)
-        a = ops.decompress(db.packed, n_sym, lo)
-        an = a.cpu().numpy()
-        D, starts, lengths, off = orc.segments(an)
-        s = torch.zeros(lengths.size, dtype=torch.float16, device=a.device)
-        maxlen = int(lengths.max())
-        t = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(a.device)
-        ops.warp_corr(s, q.qrev, q.lut_a, q.lut_b, a, t(off[:-1].astype(np.int32)),
-                      t(lengths.astype(np.int32)), q.k, len(query), maxlen,
-                      _nextpow2(max(len(query), maxlen)), torch.zeros(1, dtype=torch.float16,
-                                                                       device=a.device))
-        dev_scores.append(s.float().cpu().numpy())
-    want, fft64 = orc.batch_cascade(world["ascii"], query, bs * 8 // 5, top_fft=50, top_sw=5,
-                                    top_sw_affine=2, fft_scores=fft_scores)
-    for s, w in zip(dev_scores, fft64):
+    dev_scores = {}
+    got = batch_search(db, query, q, bs=bs, resident=resident,
+                       on_scores=lambda j, s: dev_scores.__setitem__(j, s.float().cpu().numpy()),
+                       **kw)
+    want, fft64 = orc.batch_cascade(ascii, query, bs * 8 // 5,
+                                    fft_scores=lambda j: dev_scores[j], **kw)
+    nonempty = [j for j, w in enumerate(fft64) if w.size]
+    assert sorted(dev_scores) == nonempty
+    for j in nonempty:
+        w = fft64[j]
         scale = 1e-3 * np.max(np.abs(w))
-        assert np.all(np.abs(s - w) <= 1e-2 * np.abs(w) + scale)
+        assert np.all(np.abs(dev_scores[j] - w) <= 1e-2 * np.abs(w) + scale), j
     assert len(got) == len(want) > 0
     for g, w in zip(got, want):
         assert (np.float32(g.norm), g.affine, g.linear, g.seq) == w
+    return got
+
+
+@pytest.mark.parametrize("resident", [True, False])
+def test_batch_cascade(orc, world, resident):
+    announce(f"batch cascade resident={resident}")
+    bs = 1_000_000  # bytes -> 1.6 M symbols per batch, several batches
+    _check_batch_cascade(orc, world["db"], world["ascii"], world["queries"][0], bs, resident,
+                         top_fft=50, top_sw=5, top_sw_affine=2)
+
+
+def test_batch_cascade_shifted_segments(orc, dev):
+    """Batches that start on '@' or hold '@@': the reference's `lengths != 0`
+    filter shifts the offsets of later segments; both paths reproduce it."""
+    from sseqs_amd.db import SeqDB
+    announce("batch cascade with shifted segments")
+    rng = np.random.default_rng(11)
+    aa = np.frombuffer(b"ARNDCQEGHILKMFPSTWYV", dtype=np.uint8)
+    query = rng.choice(aa, 120).tobytes().decode()
+    parts = []
+    for i in range(5000):
+        if i % 40 == 0:
+            s = np.frombuffer(mutate(rng, query, sub=0.4).encode(), dtype=np.uint8)
+        else:
+            s = rng.choice(aa, int(rng.integers(20, 280)))
+        parts.append(s.tobytes())
+        parts.append(b"@@@" if rng.random() < 0.01 else b"@")
+    a = np.frombuffer(b"".join(parts), dtype=np.uint8).copy()
+    bs = 20_000
+    sb = bs * 8 // 5
+    for b in range(1, a.size // sb, 3):  # some windows start on '@', some on '@@'
+        a[b * sb] = 64
+        if b % 2:
+            a[b * sb + 1] = 64
+    packed = torch.from_numpy(orc.xbit_pack(a)).to(dev)
+    db = SeqDB.from_packed(packed, a.size)
+    plan = db.batch_plan(bs)
+    assert (~plan.regular & (plan.nseg > 0)).any() and plan.regular.any()
+    kw = dict(top_fft=10, top_sw=3, top_sw_affine=2)
+    got_r = _check_batch_cascade(orc, db, a, query, bs, True, **kw)
+    got_a = _check_batch_cascade(orc, db, a, query, bs, False, **kw)
+    assert len(got_r) == len(got_a)
 
 
 def test_msa_global_a3m(orc, world, tmp_path):
