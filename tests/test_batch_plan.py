@@ -81,3 +81,26 @@ def test_plan_rejects_unaligned_batches():
     ln = torch.tensor([3], dtype=torch.int32)
     with pytest.raises(ValueError):
         plan_batches(off, ln, 3, 7)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_oracle_segments_are_split_lengths(orc, seed):
+    """The oracle's delimiter arithmetic (sseqs/msa.py:227-267) gives the
+    lengths of str.split('@') without its empty parts (a trailing '@' adds
+    no part), and offsets that are the running sum of (length + 1)."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(1, 400))
+    a = rng.choice(np.frombuffer(b"@@@ARNDWY", dtype=np.uint8), n)
+    if seed == 0:
+        a[0] = 64
+    if seed == 1:
+        a[-1] = 64
+    s = a.tobytes().decode()
+    parts = s.split("@")
+    if s.endswith("@"):
+        parts = parts[:-1]
+    want = [len(p) for p in parts if len(p)]
+    D, starts, lengths, fft_off = orc.segments(a)
+    assert lengths.tolist() == want
+    assert np.array_equal(D, [i for i, c in enumerate(s) if c == "@"])
+    assert fft_off.tolist() == [0] + np.cumsum(np.array(want) + 1).tolist()
