@@ -127,6 +127,8 @@ def main() -> int:
     ap.add_argument("--no-q500", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=2,
                     help="timed passes of the per-batch cascade (0 skips it)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the per-batch cascade measurement (for traces)")
     ap.add_argument("--prefilter-only", action="store_true",
                     help="run only the q_len=200 prefilter sweep --steps times (for counters)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -284,6 +286,9 @@ def main() -> int:
         log(json.dumps(res))
         return res
 
+    if args.batch_only:
+        measure_batch(q200)
+        return 0
     r200, launches200 = measure(q200)
     rbatch = measure_batch(q200) if args.batch_steps > 0 else None
     r500 = r500l = None

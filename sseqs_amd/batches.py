@@ -76,8 +76,27 @@ def plan_batches(seq_off: torch.Tensor, seq_len: torch.Tensor, n_syms: int, bs: 
                                                                         device=dev)
     base_t = torch.cumsum(nseg_t, 0) - nseg_t
     maxlen_t = torch.zeros(B, dtype=torch.int64, device=dev)
-    if total:
-        maxlen_t.scatter_reduce_(0, p_b, p_len, reduce="amax", include_self=True)
+    if total and dev.type == "cuda":
+        # longest segment of each batch: a top-1 per batch in one launch
+        from . import ops
+        nseg_h = nseg_t.cpu().numpy()
+        base_h = base_t.cpu().numpy()
+        has_h = nseg_h > 0
+        p32 = p_len.to(torch.int32)
+        top, _ = ops.topk_segments(p32, base_h[has_h], nseg_h[has_h], np.ones(int(has_h.sum())),
+                                   add_begin=True)
+        maxlen_t[torch.from_numpy(np.nonzero(has_h)[0]).to(dev)] = p32[top].to(torch.int64)
+        del p32
+    elif total:
+        # segmented max by a running max of (batch, length) keys: batches are
+        # contiguous and in order, so the key's running max at a batch's last
+        # segment holds that batch's longest segment
+        run_max = torch.cummax((p_b << 32) + p_len, 0).values
+        has_t = nseg_t > 0
+        last = (base_t + nseg_t - 1).clamp(min=0)
+        bidx = torch.arange(B, device=dev, dtype=torch.int64)
+        maxlen_t = torch.where(has_t, run_max[last] - (bidx << 32), maxlen_t)
+        del run_max
     # regular: the first segment starts the window, and segments of one
     # batch are one delimiter apart
     has = nseg_t > 0

@@ -232,6 +232,7 @@ __global__ __launch_bounds__(1024) void k_sel_sort(const void* s, int dtype, con
 constexpr int SEG_THREADS = 1024;
 constexpr int SEG_WAVES = SEG_THREADS / 64;
 constexpr int SEG_SORT_MAX = 4096;
+constexpr int SEG_E = 8;  // elements per thread per pass step
 
 // 16-bit order key of an fp16 score (NaN lowest, as key_f32)
 __device__ inline uint32_t key_h16(uint16_t h) {
@@ -251,7 +252,7 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_topk(
     int drop_back, int add_begin, int64_t* __restrict__ out) {
     __shared__ uint32_t hist[SEL_BINS];
     __shared__ uint64_t comp[SEG_SORT_MAX];
-    __shared__ int wg[SEG_WAVES], we[SEG_WAVES];
+    __shared__ uint32_t wsum[SEG_WAVES];
     __shared__ uint32_t s_pick[2];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int npass = dtype == SSEQS_F16 ? 2 : 3;
@@ -275,9 +276,20 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_topk(
             const uint32_t bm = (1u << bits) - 1u;
             for (int i = tid; i < SEL_BINS; i += SEG_THREADS) hist[i] = 0;
             __syncthreads();
-            for (int i = tid; i < n; i += SEG_THREADS) {
-                const uint32_t key = seg_key(base, dtype, i);
-                if ((key & mask) == prefix) atomicAdd(&hist[(key >> shift) & bm], 1u);
+            // SEG_E independent loads in flight per thread
+            for (int i0 = 0; i0 < n; i0 += SEG_THREADS * SEG_E) {
+                uint32_t key[SEG_E];
+#pragma unroll
+                for (int e = 0; e < SEG_E; ++e) {
+                    const int i = i0 + e * SEG_THREADS + tid;
+                    key[e] = i < n ? seg_key(base, dtype, i) : 0u;
+                }
+#pragma unroll
+                for (int e = 0; e < SEG_E; ++e) {
+                    const int i = i0 + e * SEG_THREADS + tid;
+                    if (i < n && (key[e] & mask) == prefix)
+                        atomicAdd(&hist[(key[e] >> shift) & bm], 1u);
+                }
             }
             __syncthreads();
             if (wid == 0) {
@@ -307,46 +319,62 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_topk(
             krem = s_pick[1];
             __syncthreads();
         }
-        // keys > T are all taken, keys == T while fewer than krem came before
+        // keys > T are all taken, keys == T while fewer than krem came
+        // before.  Each thread takes SEG_E consecutive elements of a chunk;
+        // a block scan of the per-thread counts (> T, == T), packed as
+        // (gt << 16 | eq), gives every element its place in index order.
         const uint32_t T = prefix;
         const int keq = (int)krem;
         const int hi = k - drop_back;
         const int64_t ob = out_base[j] - drop_front;
         const int64_t add = add_begin ? beg : 0;
         int gt_run = 0, eq_run = 0;
-        for (int i0 = 0; i0 < n; i0 += SEG_THREADS) {
-            const int i = i0 + tid;
-            const uint32_t key = i < n ? seg_key(base, dtype, i) : 0u;
-            const bool g = i < n && key > T;
-            const bool e = i < n && key == T;
-            const uint64_t mg = __ballot(g), me = __ballot(e);
-            const uint64_t below = (1ull << lane) - 1ull;
-            if (lane == 0) {
-                wg[wid] = __popcll(mg);
-                we[wid] = __popcll(me);
+        for (int i0 = 0; i0 < n; i0 += SEG_THREADS * SEG_E) {
+            const int ib = i0 + tid * SEG_E;
+            uint32_t key[SEG_E];
+#pragma unroll
+            for (int e = 0; e < SEG_E; ++e) key[e] = ib + e < n ? seg_key(base, dtype, ib + e) : 0u;
+            uint32_t cnt = 0;
+#pragma unroll
+            for (int e = 0; e < SEG_E; ++e) {
+                const bool in = ib + e < n;
+                cnt += (in && key[e] > T) ? 0x10000u : 0u;
+                cnt += (in && key[e] == T) ? 1u : 0u;
             }
+            // inclusive scan over the wave, then over the waves
+            uint32_t inc = cnt;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t t = __shfl_up(inc, o, 64);
+                if (lane >= o) inc += t;
+            }
+            if (lane == 63) wsum[wid] = inc;
             __syncthreads();
-            int og = 0, oe = 0, tg = 0, te = 0;
+            uint32_t before = inc - cnt, total = 0;
+#pragma unroll
             for (int w = 0; w < SEG_WAVES; ++w) {
-                const int a = wg[w], b = we[w];
-                if (w < wid) {
-                    og += a;
-                    oe += b;
+                const uint32_t t = wsum[w];
+                before += w < wid ? t : 0u;
+                total += t;
+            }
+            int gb = gt_run + (int)(before >> 16), eb = eq_run + (int)(before & 0xffffu);
+#pragma unroll
+            for (int e = 0; e < SEG_E; ++e) {
+                const bool in = ib + e < n;
+                const bool g = in && key[e] > T;
+                const bool q = in && key[e] == T;
+                const int slot = gb + min(eb, keq);
+                if ((g || (q && eb < keq)) && slot < k) {
+                    if (sorted)
+                        comp[slot] = ((uint64_t)(~key[e]) << 32) | (uint64_t)(uint32_t)(ib + e);
+                    else if (slot >= drop_front && slot < hi)
+                        out[ob + slot] = add + ib + e;
                 }
-                tg += a;
-                te += b;
+                gb += g;
+                eb += q;
             }
-            const int gb = gt_run + og + __popcll(mg & below);
-            const int eb = eq_run + oe + __popcll(me & below);
-            const int slot = gb + min(eb, keq);
-            if ((g || (e && eb < keq)) && slot < k) {
-                if (sorted)
-                    comp[slot] = ((uint64_t)(~key) << 32) | (uint64_t)(uint32_t)i;
-                else if (slot >= drop_front && slot < hi)
-                    out[ob + slot] = add + i;
-            }
-            gt_run += tg;
-            eq_run += te;
+            gt_run += (int)(total >> 16);
+            eq_run += (int)(total & 0xffffu);
             __syncthreads();
         }
         if (sorted) {

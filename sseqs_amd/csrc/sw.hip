@@ -17,6 +17,8 @@
 // padded with a non-residue row (score -4 everywhere), which can never raise
 // the running maximum, so scores are unchanged.  Columns past the query end
 // are padded the same way.
+#include <type_traits>
+
 #include "common.h"
 
 namespace sseqs {
@@ -161,10 +163,203 @@ __global__ __launch_bounds__(64) void k_sw(Src src, const uint8_t* __restrict__ 
     }
 }
 
+// Packed 16-bit variant for queries of up to 1024 residues (TPT <= 32).
+// A target is held by the 16 lanes of one DPP row; lane l owns stripes l
+// (low halves) and l + 16 (high halves) of the 32-stripe rule, so one
+// packed instruction advances two cells, and a wave scores four targets.
+// The diagonal operand of a stripe's first column is the previous row's
+// last column of the stripe before it: row rotation by one lane gives
+// stripe l-1 (low) and l+15 (high) for lanes 1..15; lane 0 takes 0 for
+// stripe 0 and stripe 15 (lane 15, low half) for stripe 16.
+// Exactness: every H, M, X and Y is in [0, 11 * Lq] (11 = largest BLOSUM62
+// entry), and the operands subtracted from them are the gap parameters, so
+// 16-bit arithmetic is exact while 11 * 1024 + gap parameters < 32768; the
+// host sends other parameters to the 32-bit kernel.
+// The query profile ([residue][word][lane] x 4 packed pairs) is built once
+// per block of SW16_WAVES waves; each wave stages its own targets' residues.
+typedef short i16x2 __attribute__((ext_vector_type(2)));
+constexpr int SW16_WAVES = 4;
+
+__device__ inline i16x2 as_i16x2(uint32_t u) { return __builtin_bit_cast(i16x2, u); }
+__device__ inline uint32_t as_u32(i16x2 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ inline i16x2 vmax(i16x2 a, i16x2 b) { return __builtin_elementwise_max(a, b); }
+
+template <int CTRL>
+__device__ inline int sw_dpp(int v) {
+    return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
+}
+
+template <int TPT, bool AFFINE, class Src>
+__global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
+                                                         const uint8_t* __restrict__ query,
+                                                         int Lq, int64_t n, int p1, int p2,
+                                                         int32_t* __restrict__ out, Blosum bl) {
+    constexpr int NW = (TPT + 3) / 4;
+    __shared__ int8_t s_bl[400];
+    __shared__ uint4 s_prof[21 * NW * 16];
+    __shared__ uint8_t s_res[SW16_WAVES][4][SW_CH];
+
+    const int tid = threadIdx.x;
+    const int wid = tid >> 6, lane = tid & 63;
+    const int grp = lane >> 4, l = lane & 15;
+
+    for (int i = tid; i < 400; i += 64 * SW16_WAVES) s_bl[i] = bl.v[i];
+    __syncthreads();
+    for (int idx = tid; idx < 21 * NW * 16; idx += 64 * SW16_WAVES) {
+        const int c = idx / (NW * 16);
+        const int rem = idx - c * NW * 16;
+        const int w = rem >> 4, ln = rem & 15;
+        uint32_t v[4];
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+            const int k = 4 * w + b;
+            int sv[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int j = (ln + 16 * h) * TPT + k;
+                int sc = -4;
+                if (k < TPT && j < Lq && c < 20) {
+                    const int qc = query[j];
+                    sc = qc < 20 ? (int)s_bl[c * 20 + qc] : -4;
+                }
+                sv[h] = sc;
+            }
+            v[b] = (uint32_t)(uint16_t)(int16_t)sv[0] | ((uint32_t)(uint16_t)(int16_t)sv[1] << 16);
+        }
+        s_prof[idx] = make_uint4(v[0], v[1], v[2], v[3]);
+    }
+    __syncthreads();
+
+    const i16x2 zero = {0, 0};
+    // affine: open, extend; linear: p1 = gap (<= 0)
+    const i16x2 c1 = {(short)p1, (short)p1}, c2 = {(short)p2, (short)p2};
+    uint8_t* res = s_res[wid][grp];
+    for (int64_t quad = (int64_t)blockIdx.x * SW16_WAVES + wid; quad * 4 < n;
+         quad += (int64_t)gridDim.x * SW16_WAVES) {
+        const int64_t g = quad * 4 + grp;
+        int64_t start = 0;
+        int len = 0;
+        bool valid = false;
+        if (g < n) valid = src.target(g, start, len);
+        const int T = valid ? len : 0;
+        int Tmax = max(T, __shfl_xor(T, 16, 64));
+        Tmax = max(Tmax, __shfl_xor(Tmax, 32, 64));
+
+        i16x2 Hp[TPT], Mp[TPT], Yp[TPT];
+#pragma unroll
+        for (int k = 0; k < TPT; ++k) {
+            Hp[k] = zero;
+            Mp[k] = zero;
+            Yp[k] = zero;
+        }
+        i16x2 best = zero;
+
+        for (int c0 = 0; c0 < Tmax; c0 += SW_CH) {
+            const int rows = min(SW_CH, Tmax - c0);
+            // this wave's LDS reads of the previous rows are done before
+            // the rows are overwritten, and the writes before the reads
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (int r = l; r < rows; r += 16) {
+                const int p = c0 + r;
+                res[r] = (uint8_t)(p < T ? aa_index(src.sym(start, p)) : 20);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            for (int r = 0; r < rows; ++r) {
+                const int c = res[r];
+                const uint4* pr = &s_prof[c * NW * 16 + l];
+                const uint32_t rot = (uint32_t)sw_dpp<0x121>((int)as_u32(Hp[TPT - 1]));
+                i16x2 dg = as_i16x2(l == 0 ? rot << 16 : rot);
+                if constexpr (AFFINE) {
+                    i16x2 lM = zero, lX = zero;
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) {
+                        const uint4 pw = pr[w * 16];
+                        const uint32_t pv[4] = {pw.x, pw.y, pw.z, pw.w};
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) {
+                            const int k = 4 * w + b;
+                            if (k < TPT) {
+                                const i16x2 sc = as_i16x2(pv[b]);
+                                const i16x2 m = vmax(dg + sc, zero);
+                                const i16x2 x = vmax(vmax(lM - c1, lX - c2), zero);
+                                const i16x2 y = vmax(vmax(Mp[k] - c1, Yp[k] - c2), zero);
+                                const i16x2 h = vmax(m, vmax(x, y));
+                                dg = Hp[k];
+                                Hp[k] = h;
+                                Mp[k] = m;
+                                Yp[k] = y;
+                                lM = m;
+                                lX = x;
+                                best = vmax(best, h);
+                            }
+                        }
+                    }
+                } else {
+                    i16x2 left = zero;
+#pragma unroll
+                    for (int w = 0; w < NW; ++w) {
+                        const uint4 pw = pr[w * 16];
+                        const uint32_t pv[4] = {pw.x, pw.y, pw.z, pw.w};
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) {
+                            const int k = 4 * w + b;
+                            if (k < TPT) {
+                                const i16x2 sc = as_i16x2(pv[b]);
+                                const i16x2 h =
+                                    vmax(vmax(dg + sc, Hp[k] + c1), vmax(left + c1, zero));
+                                dg = Hp[k];
+                                Hp[k] = h;
+                                left = h;
+                                best = vmax(best, h);
+                            }
+                        }
+                    }
+                }
+            }
+        }
+        int bv = max((int)best.x, (int)best.y);
+        bv = max(bv, sw_dpp<0x128>(bv));
+        bv = max(bv, sw_dpp<0x124>(bv));
+        bv = max(bv, sw_dpp<0x122>(bv));
+        bv = max(bv, sw_dpp<0x121>(bv));
+        if (l == 0 && g < n) out[g] = valid ? bv : -1;
+    }
+}
+
+// The packed kernel is exact for these parameters (see k_sw16).
+static bool sw16_ok(int Lq, bool affine, int p1, int p2) {
+    if (Lq < 1 || Lq > 1024 || getenv("SSEQS_SW_INT32")) return false;
+    if (affine) return p1 >= 0 && p2 >= 0 && p1 <= 16384 && p2 <= 16384;
+    return p1 <= 0 && p1 >= -16384;
+}
+
 template <bool AFFINE, class Src>
 static int launch_sw(const Src& src, const uint8_t* q, int Lq, int64_t n, int p1, int p2,
                      int32_t* out, hipStream_t stream) {
     const Blosum bl = blosum62();
+    if constexpr (std::is_same<Src, SrcPacked>::value) {
+        if (sw16_ok(Lq, AFFINE, p1, p2)) {
+            const int64_t quads = ceil_div(n, 4 * SW16_WAVES);
+            const unsigned grid16 = (unsigned)(quads < 8192 ? quads : 8192);
+            switch ((Lq + 31) >> 5) {
+#define SW16_CASE(T)                                                                            \
+    case T:                                                                                     \
+        hipLaunchKernelGGL((k_sw16<T, AFFINE, Src>), dim3(grid16), dim3(64 * SW16_WAVES), 0,   \
+                           stream, src, q, Lq, n, p1, p2, out, bl);                             \
+        break;
+                SW16_CASE(1) SW16_CASE(2) SW16_CASE(3) SW16_CASE(4) SW16_CASE(5) SW16_CASE(6)
+                SW16_CASE(7) SW16_CASE(8) SW16_CASE(9) SW16_CASE(10) SW16_CASE(11) SW16_CASE(12)
+                SW16_CASE(13) SW16_CASE(14) SW16_CASE(15) SW16_CASE(16) SW16_CASE(17)
+                SW16_CASE(18) SW16_CASE(19) SW16_CASE(20) SW16_CASE(21) SW16_CASE(22)
+                SW16_CASE(23) SW16_CASE(24) SW16_CASE(25) SW16_CASE(26) SW16_CASE(27)
+                SW16_CASE(28) SW16_CASE(29) SW16_CASE(30) SW16_CASE(31) SW16_CASE(32)
+#undef SW16_CASE
+            }
+            return check_launch(AFFINE ? "sw affine" : "sw linear");
+        }
+    }
     int64_t pairs = ceil_div(n, 2);
     unsigned grid = (unsigned)(pairs < 16384 ? pairs : 16384);
     const int tpt = (Lq + 31) >> 5;

@@ -156,6 +156,42 @@ def test_sw_kernels_match_oracle(orc, dev, Lq):
     assert np.array_equal(got, orc.sw_affine(qc, gi2, a, starts))
 
 
+@pytest.mark.parametrize("Lq", [1, 2, 15, 16, 17, 32, 33, 100, 200, 479, 500, 513, 1023, 1024,
+                                1025, 2048])
+def test_db_sw_matches_oracle(orc, dev, Lq, monkeypatch):
+    """Resident-database SW (packed 16-bit kernel up to 1024 residues, 32-bit
+    beyond and for gap parameters out of its range) equals the oracle, and
+    the two kernels agree; candidates in random order, count not a multiple
+    of the targets a wave holds."""
+    from sseqs_amd.db import SeqDB
+    from sseqs_amd.query import prepare
+    from sseqs_amd.search import sw_affine_ids, sw_linear_ids
+    announce(f"db sw Lq={Lq}")
+    rng = np.random.default_rng(1000 + Lq)
+    query = rand_seq(rng, Lq)
+    seqs = [mutate(rng, query) if i % 6 == 0 else rand_seq(rng, rng.integers(1, 700), DB_ALPH)
+            for i in range(203)]
+    text = ("@".join(seqs) + "@").encode()
+    db = SeqDB.from_packed(_packed_db(dev, text), len(text))
+    q = prepare(query)
+    ids = torch.from_numpy(rng.permutation(db.n)[:199]).to(dev)
+    fid = ids.cpu().numpy() if db.orig_id is None else db.orig_id.cpu().numpy()[ids.cpu().numpy()]
+    ascii = np.frombuffer(text, dtype=np.uint8)
+    starts = np.concatenate([[0], np.cumsum([len(x) + 1 for x in seqs])[:-1]])[fid]
+    lens = np.array([len(x) for x in seqs])[fid]
+    qc = orc.encode_query_sw(query)
+    for o, e in ((11, 1), (0, 0), (3, 2), (20000, 1)):
+        want = orc.sw_affine_segments(qc, ascii, starts, lens, o, e)
+        got = sw_affine_ids(db, q, ids, o, e).cpu().numpy()
+        assert np.array_equal(got, want), (o, e)
+        monkeypatch.setenv("SSEQS_SW_INT32", "1")
+        assert np.array_equal(sw_affine_ids(db, q, ids, o, e).cpu().numpy(), want), (o, e)
+        monkeypatch.delenv("SSEQS_SW_INT32")
+    for gap in (-1, -4, 0, -20000):
+        want = orc.sw_linear_segments(qc, ascii, starts, lens, gap)
+        assert np.array_equal(sw_linear_ids(db, q, ids, gap).cpu().numpy(), want), gap
+
+
 def test_sw_edge_cases(orc, dev):
     from sseqs_amd import ops
     q = orc.encode_query_sw("WCWHW")
