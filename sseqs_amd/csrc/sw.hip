@@ -173,7 +173,8 @@ __global__ __launch_bounds__(64) void k_sw(Src src, const uint8_t* __restrict__ 
 // stripe 0 and stripe 15 (lane 15, low half) for stripe 16.
 // Exactness: every H, M, X and Y is in [0, 11 * Lq] (11 = largest BLOSUM62
 // entry), and the operands subtracted from them are the gap parameters, so
-// 16-bit arithmetic is exact while 11 * 1024 + gap parameters < 32768; the
+// 16-bit arithmetic is exact while 11 * 1024 + gap parameters < 32768 (the
+// unclamped M and X of the affine update stay above -4 - 2 * 16000); the
 // host sends other parameters to the 32-bit kernel.
 // The query profile ([residue][word][lane] x 4 packed pairs) is built once
 // per block of SW16_WAVES waves; each wave stages its own targets' residues.
@@ -281,9 +282,13 @@ __global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
                         for (int b = 0; b < 4; ++b) {
                             const int k = 4 * w + b;
                             if (k < TPT) {
+                                // m and x without their clamp at 0: m >= -4, and
+                                // x >= m_left - open, so neither drifts; only
+                                // max(m, x, y) with y >= 0 is used, which equals
+                                // the clamped recurrence's H
                                 const i16x2 sc = as_i16x2(pv[b]);
-                                const i16x2 m = vmax(dg + sc, zero);
-                                const i16x2 x = vmax(vmax(lM - c1, lX - c2), zero);
+                                const i16x2 m = dg + sc;
+                                const i16x2 x = vmax(lM - c1, lX - c2);
                                 const i16x2 y = vmax(vmax(Mp[k] - c1, Yp[k] - c2), zero);
                                 const i16x2 h = vmax(m, vmax(x, y));
                                 dg = Hp[k];
@@ -307,8 +312,9 @@ __global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
                             const int k = 4 * w + b;
                             if (k < TPT) {
                                 const i16x2 sc = as_i16x2(pv[b]);
+                                // the chain through `left` is three instructions
                                 const i16x2 h =
-                                    vmax(vmax(dg + sc, Hp[k] + c1), vmax(left + c1, zero));
+                                    vmax(vmax(dg + sc, zero), vmax(Hp[k], left) + c1);
                                 dg = Hp[k];
                                 Hp[k] = h;
                                 left = h;
@@ -331,8 +337,8 @@ __global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
 // The packed kernel is exact for these parameters (see k_sw16).
 static bool sw16_ok(int Lq, bool affine, int p1, int p2) {
     if (Lq < 1 || Lq > 1024 || getenv("SSEQS_SW_INT32")) return false;
-    if (affine) return p1 >= 0 && p2 >= 0 && p1 <= 16384 && p2 <= 16384;
-    return p1 <= 0 && p1 >= -16384;
+    if (affine) return p1 >= 0 && p2 >= 0 && p1 <= 16000 && p2 <= 16000;
+    return p1 <= 0 && p1 >= -16000;
 }
 
 template <bool AFFINE, class Src>
