@@ -383,7 +383,10 @@ def batch_cascade(db_ascii: np.ndarray, query: str, bs_syms: int, top_fft=200, t
             fft64.append(np.zeros(0))
             continue
         maxlen = int(lengths.max())
-        n = nextpow2(max(Lq, maxlen))
+        # sseqs/msa.py:261; sizes outside 16..2048 (scored 0 by the
+        # reference's kernels, sseqs/filter.cu:313-390) are clamped, longer
+        # segments scored on their first 2048 residues
+        n = min(max(nextpow2(max(Lq, maxlen)), 16), 2048)
         s64 = corr(qrev, A, Bt, k, a, off[:-1], lengths, maxlen, n)
         fft64.append(s64)
         s = fft_scores(j) if fft_scores is not None else s64.astype(np.float16)

@@ -210,6 +210,18 @@ def _nextpow2(x: int) -> int:
     return 1 << (x - 1).bit_length()
 
 
+# Correlation sizes of the per-batch prefilter.  The reference's size is
+# nextpow2(max(Lq, batch maxlen)) (sseqs/msa.py:261); its kernels only run
+# for 16..1024 and leave the scores of other batches at zero
+# (sseqs/filter.cu:313-390).  Here the size is clamped to 16..2048 and
+# longer segments are scored on their first 2048 residues.
+BATCH_N_MIN, BATCH_N_MAX = 16, 2048
+
+
+def batch_corr_size(Lq: int, maxlen: int) -> int:
+    return min(max(_nextpow2(max(Lq, maxlen, 1)), BATCH_N_MIN), BATCH_N_MAX)
+
+
 def batch_search(db: SeqDB, query: str, q: PreparedQuery, bs: int = 50_000_000,
                  top_fft: int = 200, top_sw: int = 10, top_sw_affine: int = 2,
                  dlength: float = 1e6, chunk_bytes: int = 1_000_000_000,
@@ -285,7 +297,7 @@ def _ascii_batch(db: SeqDB, query: str, q: PreparedQuery, lo: int, bs: int, top_
         return []
     t0 = time.time()
     maxlen = int(lengths.max().item())
-    n = _nextpow2(max(Lq, maxlen))
+    n = batch_corr_size(Lq, maxlen)
     lengths32 = lengths.to(torch.int32)
     fft_off = torch.zeros(lengths.numel() + 1, dtype=torch.int64, device=dev)
     fft_off[1:] = torch.cumsum(lengths + 1, 0)
@@ -352,11 +364,7 @@ def _resident_batches(db: SeqDB, query: str, q: PreparedQuery, plan, top_fft: in
     active = (plan.nseg > 0) & ~(np.abs(Lq - maxl.astype(np.float64)) > dlength)
     fast = active & plan.regular
     slow = np.nonzero(active & ~plan.regular)[0]
-    Ns = np.array([_nextpow2(max(Lq, int(m))) for m in maxl], dtype=np.int64)
-    bad = fast & ((Ns < 16) | (Ns > 2048))
-    if bad.any():
-        raise ValueError(f"warp_corr: n must be a power of two in 16..2048, got "
-                         f"{int(Ns[np.nonzero(bad)[0][0]])}")
+    Ns = np.array([batch_corr_size(Lq, int(m)) for m in maxl], dtype=np.int64)
     fb = np.nonzero(fast)[0]
 
     # prefilter: one launch per run of consecutive batches with one N
@@ -372,7 +380,7 @@ def _resident_batches(db: SeqDB, query: str, q: PreparedQuery, plan, top_fft: in
         s_end = int(plan.seg_base[b1] + plan.nseg[b1])
         call("sseqs_db_prefilter", ptr(db.packed), ptr(plan.seg_off), ptr(plan.seg_len), s_begin,
              s_end, ptr(q.qrev), Lq, ptr(q.lut_a), ptr(q.lut_b_flat), 21, q.k, int(Ns[b0]),
-             int(maxl[b0:b1 + 1].max()), sc32.data_ptr() + 4 * s_begin, st)
+             min(int(maxl[b0:b1 + 1].max()), BATCH_N_MAX), sc32.data_ptr() + 4 * s_begin, st)
         i = e + 1
     sc16 = sc32.half()
     del sc32

@@ -172,6 +172,35 @@ def test_batch_cascade_shifted_segments(orc, dev):
     assert len(got_r) == len(got_a)
 
 
+def test_batch_cascade_long_segments(orc, dev):
+    """Batches whose longest segment needs a correlation size above 2048 are
+    scored at 2048 on the first 2048 residues (the reference's kernels leave
+    such batches at zero); both paths against the oracle."""
+    from sseqs_amd.db import SeqDB
+    announce("batch cascade with segments longer than 2048")
+    rng = np.random.default_rng(12)
+    aa = np.frombuffer(b"ARNDCQEGHILKMFPSTWYV", dtype=np.uint8)
+    query = rng.choice(aa, 150).tobytes().decode()
+    parts = []
+    for i in range(1600):
+        if i % 50 == 7:
+            s = np.frombuffer(mutate(rng, query * 20, sub=0.3).encode(), dtype=np.uint8)
+        elif i % 25 == 0:
+            s = np.frombuffer(mutate(rng, query, sub=0.3).encode(), dtype=np.uint8)
+        else:
+            s = rng.choice(aa, int(rng.integers(20, 400)))
+        parts.append(s.tobytes())
+        parts.append(b"@")
+    a = np.frombuffer(b"".join(parts), dtype=np.uint8).copy()
+    db = SeqDB.from_packed(torch.from_numpy(orc.xbit_pack(a)).to(dev), a.size)
+    bs = 50_000
+    plan = db.batch_plan(bs)
+    assert plan.maxlen.max() > 2048
+    kw = dict(top_fft=10, top_sw=3, top_sw_affine=2)
+    _check_batch_cascade(orc, db, a, query, bs, True, **kw)
+    _check_batch_cascade(orc, db, a, query, bs, False, **kw)
+
+
 def test_msa_global_a3m(orc, world, tmp_path):
     import sseqs_amd
     announce("msa global a3m")
