@@ -26,9 +26,10 @@ __device__ inline uint32_t key_f32(float x) {
     return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
 }
 
-__device__ inline uint32_t load_key(const void* s, int dtype, int64_t i) {
-    if (dtype == SSEQS_F32) return key_f32(static_cast<const float*>(s)[i]);
-    if (dtype == SSEQS_F16) return key_f32((float)static_cast<const _Float16*>(s)[i]);
+template <int DT>
+__device__ inline uint32_t load_key(const void* s, int64_t i) {
+    if constexpr (DT == SSEQS_F32) return key_f32(static_cast<const float*>(s)[i]);
+    if constexpr (DT == SSEQS_F16) return key_f32((float)static_cast<const _Float16*>(s)[i]);
     return (uint32_t) static_cast<const int32_t*>(s)[i] ^ 0x80000000u;
 }
 
@@ -42,16 +43,23 @@ __global__ void k_sel_init(uint32_t* hist, SelState* st, uint32_t k) {
     }
 }
 
-__global__ __launch_bounds__(256) void k_sel_hist(const void* s, int dtype, int64_t n,
+template <int DT>
+__global__ __launch_bounds__(256) void k_sel_hist(const void* s, int64_t n,
                                                   const SelState* st, uint32_t* hist, int shift,
                                                   int bits) {
     __shared__ uint32_t lh[SEL_BINS];
     for (int i = threadIdx.x; i < SEL_BINS; i += 256) lh[i] = 0;
     __syncthreads();
     const uint32_t prefix = st->prefix, mask = st->mask, bm = (1u << bits) - 1u;
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) {
-        uint32_t key = load_key(s, dtype, i);
-        if ((key & mask) == prefix) atomicAdd(&lh[(key >> shift) & bm], 1u);
+    // four unconditional (index-clamped) loads in flight per thread
+    for (int64_t i0 = (int64_t)blockIdx.x * 1024; i0 < n; i0 += (int64_t)gridDim.x * 1024) {
+        uint32_t key[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) key[e] = load_key<DT>(s, min(i0 + e * 256 + threadIdx.x, n - 1));
+#pragma unroll
+        for (int e = 0; e < 4; ++e)
+            if (i0 + e * 256 + threadIdx.x < n && (key[e] & mask) == prefix)
+                atomicAdd(&lh[(key[e] >> shift) & bm], 1u);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < SEL_BINS; i += 256)
@@ -81,7 +89,8 @@ __global__ __launch_bounds__(1024) void k_sel_pick(SelState* st, uint32_t* hist,
     }
 }
 
-__global__ __launch_bounds__(256) void k_sel_count(const void* s, int dtype, int64_t n,
+template <int DT>
+__global__ __launch_bounds__(256) void k_sel_count(const void* s, int64_t n,
                                                    int64_t nb, const SelState* st, int64_t* cgt,
                                                    int64_t* ceq) {
     const uint32_t T = st->prefix;
@@ -89,10 +98,11 @@ __global__ __launch_bounds__(256) void k_sel_count(const void* s, int dtype, int
     for (int64_t tile = blockIdx.x; tile < nb; tile += gridDim.x) {
     const int64_t base = tile * SEL_TILE;
     int gt = 0, eq = 0;
+#pragma unroll 4
     for (int j = threadIdx.x; j < SEL_TILE; j += 256) {
-        int64_t i = base + j;
+        const int64_t i = base + j;
+        const uint32_t key = load_key<DT>(s, min(i, n - 1));
         if (i < n) {
-            uint32_t key = load_key(s, dtype, i);
             gt += key > T;
             eq += key == T;
         }
@@ -146,7 +156,8 @@ __global__ __launch_bounds__(1024) void k_sel_scan(int64_t* a, int64_t* b, int64
     }
 }
 
-__global__ __launch_bounds__(256) void k_sel_write(const void* s, int dtype, int64_t n,
+template <int DT>
+__global__ __launch_bounds__(256) void k_sel_write(const void* s, int64_t n,
                                                    int64_t nb, const SelState* st,
                                                    const int64_t* bgt, const int64_t* beq,
                                                    int64_t k, int64_t* out) {
@@ -159,7 +170,7 @@ __global__ __launch_bounds__(256) void k_sel_write(const void* s, int dtype, int
     const int64_t base = tile * SEL_TILE;
     for (int j0 = 0; j0 < SEL_TILE; j0 += 256) {
         int64_t i = base + j0 + threadIdx.x;
-        uint32_t key = i < n ? load_key(s, dtype, i) : 0u;
+        const uint32_t key = load_key<DT>(s, min(i, n - 1));
         bool g = i < n && key > T;
         bool e = i < n && key == T;
         uint64_t mg = __ballot(g), me = __ballot(e);
@@ -189,13 +200,14 @@ __global__ __launch_bounds__(256) void k_sel_write(const void* s, int dtype, int
 }
 
 // Sort the k selected indices by (key desc, index asc); k <= SEL_SORT_MAX.
-__global__ __launch_bounds__(1024) void k_sel_sort(const void* s, int dtype, const int64_t* sel,
+template <int DT>
+__global__ __launch_bounds__(1024) void k_sel_sort(const void* s, const int64_t* sel,
                                                    int64_t k, int P, int64_t* out) {
     extern __shared__ uint64_t comp[];
     for (int i = threadIdx.x; i < P; i += 1024) {
         if (i < k) {
             int64_t idx = sel[i];
-            uint32_t key = load_key(s, dtype, idx);
+            uint32_t key = load_key<DT>(s, idx);
             comp[i] = ((uint64_t)(~key) << 32) | (uint64_t)(uint32_t)idx;
         } else {
             comp[i] = ~0ull;
@@ -240,13 +252,17 @@ __device__ inline uint32_t key_h16(uint16_t h) {
     return (h & 0x8000u) ? (~h & 0xffffu) : (h | 0x8000u);
 }
 
-__device__ inline uint32_t seg_key(const void* s, int dtype, int64_t i) {
-    if (dtype == SSEQS_F16) return key_h16(static_cast<const uint16_t*>(s)[i]);
-    return load_key(s, dtype, i);
+template <int DT>
+__device__ inline uint32_t seg_key(const void* s, int64_t i) {
+    if constexpr (DT == SSEQS_F16)
+        return key_h16(static_cast<const uint16_t*>(s)[i]);
+    else
+        return load_key<DT>(s, i);
 }
 
+template <int DT>
 __global__ __launch_bounds__(SEG_THREADS) void k_seg_topk(
-    const void* __restrict__ s, int dtype, const int64_t* __restrict__ seg_begin,
+    const void* __restrict__ s, const int64_t* __restrict__ seg_begin,
     const int32_t* __restrict__ seg_n, const int32_t* __restrict__ seg_k,
     const int64_t* __restrict__ out_base, int64_t nseg, int sorted, int drop_front,
     int drop_back, int add_begin, int64_t* __restrict__ out) {
@@ -255,7 +271,7 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_topk(
     __shared__ uint32_t wsum[SEG_WAVES];
     __shared__ uint32_t s_pick[2];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    const int npass = dtype == SSEQS_F16 ? 2 : 3;
+    const int npass = DT == SSEQS_F16 ? 2 : 3;
     const int shifts16[2] = {5, 0}, bits16[2] = {11, 5};
     const int shifts32[3] = {21, 10, 0}, bits32[3] = {11, 11, 10};
     for (int64_t j = blockIdx.x; j < nseg; j += gridDim.x) {
@@ -264,15 +280,15 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_topk(
         int k = min(seg_k[j], n);
         if (sorted) k = min(k, SEG_SORT_MAX);
         if (k <= 0) continue;  // uniform over the block
-        const void* base = dtype == SSEQS_F16
+        const void* base = DT == SSEQS_F16
                                ? static_cast<const void*>(static_cast<const uint16_t*>(s) + beg)
-                           : dtype == SSEQS_F32
+                           : DT == SSEQS_F32
                                ? static_cast<const void*>(static_cast<const float*>(s) + beg)
                                : static_cast<const void*>(static_cast<const int32_t*>(s) + beg);
         uint32_t prefix = 0, mask = 0, krem = (uint32_t)k;
         for (int p = 0; p < npass; ++p) {
-            const int shift = dtype == SSEQS_F16 ? shifts16[p] : shifts32[p];
-            const int bits = dtype == SSEQS_F16 ? bits16[p] : bits32[p];
+            const int shift = DT == SSEQS_F16 ? shifts16[p] : shifts32[p];
+            const int bits = DT == SSEQS_F16 ? bits16[p] : bits32[p];
             const uint32_t bm = (1u << bits) - 1u;
             for (int i = tid; i < SEL_BINS; i += SEG_THREADS) hist[i] = 0;
             __syncthreads();
@@ -281,8 +297,9 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_topk(
                 uint32_t key[SEG_E];
 #pragma unroll
                 for (int e = 0; e < SEG_E; ++e) {
-                    const int i = i0 + e * SEG_THREADS + tid;
-                    key[e] = i < n ? seg_key(base, dtype, i) : 0u;
+                    // clamped index: the loads are unconditional and all in flight
+                    const int i = min(i0 + e * SEG_THREADS + tid, n - 1);
+                    key[e] = seg_key<DT>(base, i);
                 }
 #pragma unroll
                 for (int e = 0; e < SEG_E; ++e) {
@@ -333,7 +350,7 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_topk(
             const int ib = i0 + tid * SEG_E;
             uint32_t key[SEG_E];
 #pragma unroll
-            for (int e = 0; e < SEG_E; ++e) key[e] = ib + e < n ? seg_key(base, dtype, ib + e) : 0u;
+            for (int e = 0; e < SEG_E; ++e) key[e] = seg_key<DT>(base, min(ib + e, n - 1));
             uint32_t cnt = 0;
 #pragma unroll
             for (int e = 0; e < SEG_E; ++e) {
@@ -409,6 +426,17 @@ __global__ __launch_bounds__(SEG_THREADS) void k_seg_topk(
 
 using namespace sseqs;
 
+// Launch the instance of a select kernel for the score type `dtype`.
+#define SEL_DISPATCH(KERNEL, ...)                                                   \
+    do {                                                                          \
+        if (dtype == SSEQS_F32)                                                   \
+            hipLaunchKernelGGL(KERNEL<SSEQS_F32>, __VA_ARGS__);                   \
+        else if (dtype == SSEQS_F16)                                              \
+            hipLaunchKernelGGL(KERNEL<SSEQS_F16>, __VA_ARGS__);                   \
+        else                                                                      \
+            hipLaunchKernelGGL(KERNEL<SSEQS_I32>, __VA_ARGS__);                   \
+    } while (0)
+
 extern "C" size_t sseqs_topk_ws_bytes(int64_t n) {
     int64_t nb = ceil_div(n > 0 ? n : 1, SEL_TILE);
     return (size_t)SEL_BINS * 4 + 64 + (size_t)nb * 16 + (size_t)SEL_SORT_MAX * 8 + 256;
@@ -438,30 +466,33 @@ extern "C" int sseqs_topk(const void* scores, int dtype, int64_t n, int64_t k, i
     int64_t hb = ceil_div(n, 256 * 16);
     unsigned hgrid = (unsigned)(hb < 4096 ? (hb > 0 ? hb : 1) : 4096);
     for (int p = 0; p < 3; ++p) {
-        hipLaunchKernelGGL(k_sel_hist, dim3(hgrid), dim3(256), 0, stream, scores, dtype, n, st, hist,
+        SEL_DISPATCH(k_sel_hist, dim3(hgrid), dim3(256), 0, stream, scores, n, st, hist,
                            shifts[p], bits[p]);
         if (int r = check_launch("topk hist")) return r;
         hipLaunchKernelGGL(k_sel_pick, dim3(1), dim3(1024), 0, stream, st, hist, shifts[p], bits[p]);
         if (int r = check_launch("topk pick")) return r;
     }
-    hipLaunchKernelGGL(k_sel_count, dim3(grid_cap(nb, 1)), dim3(256), 0, stream, scores, dtype, n,
+    SEL_DISPATCH(k_sel_count, dim3(grid_cap(nb, 1)), dim3(256), 0, stream, scores, n,
                        nb, st, cgt, ceq);
     if (int r = check_launch("topk count")) return r;
     hipLaunchKernelGGL(k_sel_scan, dim3(1), dim3(1024), 0, stream, cgt, ceq, nb);
     if (int r = check_launch("topk scan")) return r;
     int64_t* dst = sorted ? tmp : out_idx;
-    hipLaunchKernelGGL(k_sel_write, dim3(grid_cap(nb, 1)), dim3(256), 0, stream, scores, dtype, n,
+    SEL_DISPATCH(k_sel_write, dim3(grid_cap(nb, 1)), dim3(256), 0, stream, scores, n,
                        nb, st, cgt, ceq, k, dst);
     if (int r = check_launch("topk write")) return r;
     if (sorted) {
         int P = 1;
         while (P < k) P <<= 1;
         size_t lds = (size_t)P * 8;
+        const void* sort_fn = dtype == SSEQS_F32   ? reinterpret_cast<const void*>(&k_sel_sort<SSEQS_F32>)
+                              : dtype == SSEQS_F16 ? reinterpret_cast<const void*>(&k_sel_sort<SSEQS_F16>)
+                                                   : reinterpret_cast<const void*>(&k_sel_sort<SSEQS_I32>);
         if (lds > 64 * 1024 &&
-            hipFuncSetAttribute(reinterpret_cast<const void*>(&k_sel_sort),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) != hipSuccess)
+            hipFuncSetAttribute(sort_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) !=
+                hipSuccess)
             return fail(-3, "topk: cannot enable large LDS for the sort");
-        hipLaunchKernelGGL(k_sel_sort, dim3(1), dim3(1024), lds, stream, scores, dtype, tmp, k, P,
+        SEL_DISPATCH(k_sel_sort, dim3(1), dim3(1024), lds, stream, scores, tmp, k, P,
                            out_idx);
         if (int r = check_launch("topk sort")) return r;
     }
@@ -483,7 +514,7 @@ extern "C" int sseqs_topk_segments(const void* scores, int dtype, const int64_t*
     if (!scores || !seg_begin || !seg_n || !seg_k || !out_base || !out_idx)
         return fail(-1, "topk_segments: null pointer");
     const unsigned grid = (unsigned)(nseg < 8192 ? nseg : 8192);
-    hipLaunchKernelGGL(k_seg_topk, dim3(grid), dim3(SEG_THREADS), 0, stream, scores, dtype,
+    SEL_DISPATCH(k_seg_topk, dim3(grid), dim3(SEG_THREADS), 0, stream, scores,
                        seg_begin, seg_n, seg_k, out_base, nseg, sorted, drop_front, drop_back,
                        add_begin, out_idx);
     return check_launch("topk_segments");
