@@ -13,7 +13,7 @@
 namespace sseqs {
 
 constexpr int SEL_BINS = 2048;
-constexpr int SEL_TILE = 4096;      // elements per block in count/write passes
+constexpr int SEL_TILE = 16384;     // elements per block in count/write passes
 constexpr int SEL_SORT_MAX = 16384;
 
 struct SelState {
@@ -164,38 +164,51 @@ __global__ __launch_bounds__(256) void k_sel_write(const void* s, int64_t n,
     const uint32_t T = st->prefix;
     const int64_t keq = st->k_rem;
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    __shared__ int wg[4], we[4];
+    const uint64_t below = (1ull << lane) - 1ull;
+    __shared__ int wg[4][4], we[4][4];  // [step][wave]
     for (int64_t tile = blockIdx.x; tile < nb; tile += gridDim.x) {
-    int64_t gt_run = bgt[tile], eq_run = beq[tile];
-    const int64_t base = tile * SEL_TILE;
-    for (int j0 = 0; j0 < SEL_TILE; j0 += 256) {
-        int64_t i = base + j0 + threadIdx.x;
-        const uint32_t key = load_key<DT>(s, min(i, n - 1));
-        bool g = i < n && key > T;
-        bool e = i < n && key == T;
-        uint64_t mg = __ballot(g), me = __ballot(e);
-        uint64_t below = (1ull << lane) - 1ull;
-        int lg = __popcll(mg & below), le = __popcll(me & below);
-        if (lane == 0) {
-            wg[wid] = __popcll(mg);
-            we[wid] = __popcll(me);
+        int64_t gt_run = bgt[tile], eq_run = beq[tile];
+        const int64_t base = tile * SEL_TILE;
+        // four steps of 256 elements per barrier pair; loads first, in flight together
+        for (int j0 = 0; j0 < SEL_TILE; j0 += 1024) {
+            uint32_t key[4];
+            uint64_t mg[4], me[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                key[t] = load_key<DT>(s, min(base + j0 + t * 256 + threadIdx.x, n - 1));
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const bool in = base + j0 + t * 256 + threadIdx.x < n;
+                mg[t] = __ballot(in && key[t] > T);
+                me[t] = __ballot(in && key[t] == T);
+                if (lane == 0) {
+                    wg[t][wid] = __popcll(mg[t]);
+                    we[t][wid] = __popcll(me[t]);
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                int og = 0, oe = 0, tg = 0, te = 0;
+#pragma unroll
+                for (int w = 0; w < 4; ++w) {
+                    if (w < wid) {
+                        og += wg[t][w];
+                        oe += we[t][w];
+                    }
+                    tg += wg[t][w];
+                    te += we[t][w];
+                }
+                const bool g = (mg[t] >> lane) & 1ull, e = (me[t] >> lane) & 1ull;
+                const int64_t gb = gt_run + og + __popcll(mg[t] & below);  // keys > T before i
+                const int64_t eb = eq_run + oe + __popcll(me[t] & below);  // keys == T before i
+                const int64_t slot = gb + min(eb, keq);
+                if ((g || (e && eb < keq)) && slot < k) out[slot] = base + j0 + t * 256 + threadIdx.x;
+                gt_run += tg;
+                eq_run += te;
+            }
+            __syncthreads();
         }
-        __syncthreads();
-        int og = 0, oe = 0;
-        for (int w = 0; w < wid; ++w) {
-            og += wg[w];
-            oe += we[w];
-        }
-        int tg = wg[0] + wg[1] + wg[2] + wg[3];
-        int te = we[0] + we[1] + we[2] + we[3];
-        int64_t gb = gt_run + og + lg;  // keys > T before i
-        int64_t eb = eq_run + oe + le;  // keys == T before i
-        const int64_t slot = gb + min(eb, keq);
-        if ((g || (e && eb < keq)) && slot < k) out[slot] = i;
-        gt_run += tg;
-        eq_run += te;
-        __syncthreads();
-    }
     }
 }
 
