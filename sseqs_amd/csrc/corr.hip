@@ -811,24 +811,29 @@ __global__ __launch_bounds__(64, (G == 1 && R <= 8 ? 2 : 1)) void k_corr_band(
             const int pb = 32 * v + 8 * g;
             const int Le = cur.Le[gg];
             uint32_t sym[8];
+            // the 40-bit window of the 8 symbols, from the stage (fast) or
+            // straight from the packed stream (16 bytes at an 8-aligned
+            // address, inside the stream's tail padding at its end)
+            const uint8_t* wbase;
+            int64_t bit;
             if constexpr (decltype(fast_tag)::value) {
-                const int64_t bit = relbit[gg] + 5 * (int64_t)pb;
-                const int64_t byte = bit >> 3;
-                const uint64_t* s64 = reinterpret_cast<const uint64_t*>(sbuf + (byte & ~7));
-                const uint64_t B0 = __builtin_bswap64(s64[0]);
-                const uint64_t B1 = __builtin_bswap64(s64[1]);
-                const int sh = (int)(byte & 7) * 8 + (int)(bit & 7);
-                uint64_t top = (B0 << sh) | ((B1 >> 1) >> (63 - sh));
-                // keep the first min(max(Le - pb, 0), 8) symbols of the window
-                const int nv = min(max(Le - pb, 0), 8);
-                top &= ~(~0ull >> (5 * nv));
-#pragma unroll
-                for (int e = 0; e < 8; ++e) sym[e] = (uint32_t)(top >> (59 - 5 * e)) & 31u;
+                wbase = sbuf;
+                bit = relbit[gg] + 5 * (int64_t)pb;
             } else {
-#pragma unroll
-                for (int e = 0; e < 8; ++e)
-                    sym[e] = pb + e < Le ? packed_sym(packed, cur.o[gg] + pb + e) - 64u : 0u;
+                wbase = packed;
+                bit = 5 * (cur.o[gg] + (int64_t)pb);
             }
+            const int64_t byte = bit >> 3;
+            const uint64_t* s64 = reinterpret_cast<const uint64_t*>(wbase + (byte & ~7));
+            const uint64_t B0 = __builtin_bswap64(s64[0]);
+            const uint64_t B1 = __builtin_bswap64(s64[1]);
+            const int sh = (int)(byte & 7) * 8 + (int)(bit & 7);
+            uint64_t top = (B0 << sh) | ((B1 >> 1) >> (63 - sh));
+            // keep the first min(max(Le - pb, 0), 8) symbols of the window
+            const int nv = min(max(Le - pb, 0), 8);
+            top &= ~(~0ull >> (5 * nv));
+#pragma unroll
+            for (int e = 0; e < 8; ++e) sym[e] = (uint32_t)(top >> (59 - 5 * e)) & 31u;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 union {
