@@ -47,6 +47,25 @@ def test_argument_errors_need_no_gpu():
     assert lib.launch_warp_corr(None, None, None, None, None, None, None, None, 4, 21, 4, 10, 10,
                                 100, None, None) == -1
     assert b"power of two" in lib.sseqs_last_error()
+    # segmented selects: bad sizes, dtype and the sorted limit
+    assert lib.sseqs_topk_segments(None, 0, None, None, None, None, -1, 1, 0, 0, 0, 0, None,
+                                   None) == -1
+    assert lib.sseqs_topk_segments(None, 7, None, None, None, None, 3, 1, 0, 0, 0, 0, None,
+                                   None) == -1
+    assert lib.sseqs_topk_segments(None, 0, None, None, None, None, 3, 5000, 1, 0, 0, 0, None,
+                                   None) == -1
+    assert b"4096" in lib.sseqs_last_error()
+    assert lib.sseqs_topk_segments(None, 0, None, None, None, None, 3, 5, 0, 0, 0, 0, None,
+                                   None) == -1
+    assert b"null" in lib.sseqs_last_error()
+    # nothing to do is not an error
+    assert lib.sseqs_topk_segments(None, 0, None, None, None, None, 0, 5, 0, 0, 0, 0, None,
+                                   None) == 0
+    # resident-database SW: query length and gap sign
+    assert lib.sseqs_db_sw_linear(None, None, None, None, 5, None, 0, -1, None, None) == -1
+    assert lib.sseqs_db_sw_linear(None, None, None, None, 5, None, 10, 2, None, None) == -1
+    assert lib.sseqs_db_sw_affine(None, None, None, None, 5, None, 2049, 11, 1, None, None) == -1
+    assert lib.sseqs_db_sw_affine(None, None, None, None, -1, None, 20, 11, 1, None, None) == -1
 
 
 def test_prefilter_kernel_plan_needs_no_gpu():
