@@ -9,7 +9,10 @@ thread per 8-symbol group would stop at 2^32 groups = 34.4 G symbols).
   and the very last ones) equal the CPU generator's;
 * the whole-database cascade for the 500-residue query: prefilter scores of
   a random sample within tolerance of the FP64 closed form, SW scores of
-  its hits bit-exact, and the hits are the query's planted homologs.
+  its hits bit-exact, and the hits are the query's planted homologs;
+* msa()'s per-batch cascade for the 200-residue query over all 710 batches:
+  sampled prefilter scores within tolerance, and every whole hit's linear,
+  affine and normalised scores recomputed from its text.
 """
 import numpy as np
 import pytest
@@ -83,3 +86,53 @@ def test_full_cascade_q500(orc, big):
     got = gh.prefilter.cpu().numpy()[samp]
     scale = 1e-3 * np.max(np.abs(want))
     assert np.all(np.abs(got - want) <= 1e-2 * np.abs(want) + scale)
+
+
+def test_full_batch_cascade_q200(orc, big):
+    """msa()'s per-batch cascade over the full database (710 batches of
+    50 MB): the prefilter scores of sampled segments of three batches within
+    tolerance of FP64, every hit's linear and affine scores and normalised
+    score recomputed from its text (whole hits), and the hits found."""
+    from sseqs_amd.db import SeqDB
+    from sseqs_amd.query import prepare
+    from sseqs_amd.search import batch_search
+    db, query = big["db"], big["queries"][0]
+    bs = 50_000_000
+    announce("per-batch cascade q_len=200 over 200M sequences")
+    q = prepare(query)
+    keep = {0: None, 355: None, 709: None}
+
+    def grab(j, s):
+        if j in keep:
+            keep[j] = s.float().cpu().numpy()
+    hits = batch_search(db, query, q, bs=bs, on_scores=grab)
+    plan = db.batch_plan(bs)
+    assert plan.n_batches == 710 and len(hits) == 2 * 710
+    # prefilter of sampled segments of three batches against FP64
+    segs = SeqDB(db.packed, db.n_syms, plan.seg_off, plan.seg_len, None)
+    A, B = orc.weighted_luts(query, 4)
+    qrev = orc.encode_query_rev(query)
+    rng = np.random.default_rng(5)
+    for b, got in keep.items():
+        assert got is not None and got.size == plan.nseg[b]
+        loc = np.sort(rng.choice(got.size, 200, replace=False))
+        rows = segs.strings(torch.from_numpy(plan.seg_base[b] + loc).to(db.device), 1024)
+        n = min(max(orc.nextpow2(max(len(query), int(plan.maxlen[b]))), 16), 2048)
+        want = np.array([orc.corr(qrev, A, B, 4, r.encode(), [0], [len(r)], 10**6, n)[0]
+                         for r in rows])
+        scale = 1e-3 * np.max(np.abs(want))
+        assert np.all(np.abs(got[loc] - want) <= 1e-2 * np.abs(want) + scale), b
+    # hits: scores recomputed from their text where the text is whole
+    qc = orc.encode_query_sw(query)
+    checked = 0
+    for i, h in enumerate(hits):
+        b = i // 2
+        assert (h.chunk, h.batch) == (b // 20, b % 20)
+        if len(h.seq) < 500:
+            assert h.linear == orc.sw_linear_one(qc, h.seq, -1)
+            assert h.affine == orc.sw_affine_one(qc, h.seq[1:] + "@")
+            norm = np.float32(h.affine) / np.sqrt(np.float32(len(query) * int(plan.maxlen[b])))
+            assert np.float32(h.norm) == norm
+            checked += 1
+    assert checked > 500
+    assert max(h.norm for h in hits) > 4.0
