@@ -8,8 +8,10 @@ Heavy recurrences live in oracle.c (built to oracle/_oracle.so by
 oracle/Makefile); the host-side steps of the reference (LUTs, encodings,
 segmentation, the per-batch cascade, the a3m writer) are restated here in
 numpy.  Each function cites the reference file:line it follows.  Tie orders
-that the reference leaves to torch.topk / numpy.argsort are fixed here the
-same way the product fixes them: score descending, then index ascending.
+that the reference leaves to torch.topk are fixed here the same way the
+product fixes them: score descending, then index ascending; where the
+reference sorts with numpy (final merge, a3m rows) the same numpy call is
+made.
 """
 from __future__ import annotations
 
@@ -303,10 +305,10 @@ def backtrack(query: str, target: str, o=11, e=1) -> dict:
 
 
 def a3m(query: str, aligned: list[dict]) -> str:
-    """a3m text of sseqs/to_a3m.py:16-72 from backtrack dicts; score ties
-    are kept in input order (stable descending sort)."""
-    scores = np.array([a["score"] for a in aligned], dtype=np.float64)
-    order = np.argsort(-scores, kind="stable")
+    """a3m text of sseqs/to_a3m.py:16-72 from backtrack dicts, ordered by
+    np.argsort of the Python float scores, reversed (sseqs/to_a3m.py:16-17)."""
+    scores = [float(a["score"]) for a in aligned]
+    order = np.argsort(scores)[::-1]
     aligned = [aligned[i] for i in order]
     out = f">original_query\n{query}\n"
     seen = set()
@@ -411,9 +413,10 @@ def batch_cascade(db_ascii: np.ndarray, query: str, bs_syms: int, top_fft=200, t
 
 
 def rank_hits(hits, num_msas):
-    """sseqs/msa.py:426-427 with ties in collection order."""
-    s = np.array([h[0] for h in hits], dtype=np.float64)
-    return [hits[i] for i in np.argsort(-s, kind="stable")[:num_msas]]
+    """sseqs/msa.py:426-427: np.argsort of the float32 normalised scores,
+    reversed (ties as numpy orders them)."""
+    s = np.array([h[0] for h in hits], dtype=np.float32)
+    return [hits[i] for i in np.argsort(s)[::-1][:num_msas]]
 
 
 def gen_seqs(params, queries: list[str], cls_start, cls_len, ids) -> list[str]:

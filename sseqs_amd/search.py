@@ -468,10 +468,11 @@ def _resident_batches(db: SeqDB, query: str, q: PreparedQuery, plan, top_fft: in
 
 
 def rank_hits(hits: list[BatchHit], num_msas: int) -> list[BatchHit]:
-    """Final merge (sseqs/msa.py:411-427): normalised score descending, ties in
-    collection order, first num_msas."""
+    """Final merge (sseqs/msa.py:411-427): the reference's own ordering,
+    np.argsort over the float32 normalised scores reversed, so ties come out
+    as numpy orders them; first num_msas."""
     if not hits:
         return []
     s = np.array([h.norm for h in hits], dtype=np.float32)
-    order = np.argsort(-s.astype(np.float64), kind="stable")[:num_msas]
+    order = np.argsort(s)[::-1][:num_msas]
     return [hits[i] for i in order]

@@ -29,8 +29,10 @@ def build_row(query: str, ares: dict) -> str:
 
 
 def a3m_text(query: str, aligned: list[dict]) -> str:
-    scores = np.array([a["score"] for a in aligned], dtype=np.float64)
-    order = np.argsort(-scores, kind="stable")
+    # the reference's own ordering, np.argsort(scores)[::-1] over the Python
+    # float scores (sseqs/to_a3m.py:16-17), so ties come out as numpy orders them
+    scores = [float(a["score"]) for a in aligned]
+    order = np.argsort(scores)[::-1]
     out = [f">original_query\n{query}\n"]
     seen = set()
     for i, idx in enumerate(order):

@@ -407,6 +407,27 @@ def test_backtrack_matches_oracle(orc, dev):
             assert best.cpu().tolist() == [int(w["score"]) for w in want]
 
 
+def test_backtrack_and_a3m_match_reference_vectors(dev, tmp_path):
+    """sw_affine_backtrack and to_a3m against vectors of the reference's own
+    functions (tests/golden/a3m_golden.json)."""
+    import json
+    from pathlib import Path
+
+    from sseqs_amd.sw import sw_affine_backtrack
+    from sseqs_amd.to_a3m import to_a3m
+    announce("backtrack + a3m vs reference vectors")
+    golden = json.loads((Path(__file__).resolve().parent / "golden" / "a3m_golden.json").read_text())
+    for i, c in enumerate(golden):
+        _, best, aligned = sw_affine_backtrack(c["query"], c["targets"], c["gap_open"],
+                                               c["gap_extend"])
+        assert aligned == c["aligned"]
+        assert best.cpu().tolist() == c["best"]
+        fn = tmp_path / f"case{i}.a3m"
+        to_a3m(c["query"], c["targets"], str(fn), gap_open=c["gap_open"],
+               gap_extend=c["gap_extend"])
+        assert fn.read_text() == c["a3m"]
+
+
 # ------------------------------------------------------------- generator --
 
 def test_generator_matches_cpu(orc, dev):
