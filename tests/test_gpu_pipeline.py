@@ -139,6 +139,16 @@ def test_batch_cascade(orc, world, resident):
                          top_fft=50, top_sw=5, top_sw_affine=2)
 
 
+@pytest.mark.parametrize("bs,qi,top_fft,top_sw", [(250_000, 0, 30, 3), (333_335, 1, 50, 4),
+                                                  (99_995, 1, 20, 2)])
+def test_batch_cascade_configs(orc, world, bs, qi, top_fft, top_sw):
+    """Other batch sizes (batch boundaries at other places, more and smaller
+    batches), the other query and other cascade fractions."""
+    announce(f"batch cascade bs={bs} q={qi} top_fft={top_fft} top_sw={top_sw}")
+    _check_batch_cascade(orc, world["db"], world["ascii"], world["queries"][qi], bs, True,
+                         top_fft=top_fft, top_sw=top_sw, top_sw_affine=2)
+
+
 def test_batch_cascade_shifted_segments(orc, dev):
     """Batches that start on '@' or hold '@@': the reference's `lengths != 0`
     filter shifts the offsets of later segments; both paths reproduce it."""
