@@ -251,6 +251,30 @@ def test_warp_corr_matches_fp64(orc, dev, N, k):
     assert ok.all(), (N, k, np.max(np.abs(got - want)), got[~ok][:5], want[~ok][:5])
 
 
+def test_db_prefilter_sequences_longer_than_2048(orc, dev):
+    """Sequences longer than the largest correlation size share N = 2048 and
+    are scored on their first 2048 residues (whole-database prefilter)."""
+    from sseqs_amd.db import SeqDB
+    from sseqs_amd.query import prepare
+    from sseqs_amd.search import prefilter_scores
+    announce("db_prefilter with sequences longer than 2048")
+    rng = np.random.default_rng(31)
+    query = rand_seq(rng, 300)
+    seqs = [rand_seq(rng, rng.integers(1, 600)) for _ in range(300)]
+    seqs += [mutate(rng, query * 12) for _ in range(6)] + [rand_seq(rng, 5000)]
+    text = ("@".join(seqs) + "@").encode()
+    db = SeqDB.from_packed(_packed_db(dev, text), len(text))
+    assert int(db.seq_len.max().item()) > 2048
+    got = prefilter_scores(db, prepare(query)).cpu().numpy()
+    order = db.orig_id.cpu().numpy() if db.orig_id is not None else np.arange(db.n)
+    A, B = orc.weighted_luts(query, 4)
+    qrev = orc.encode_query_rev(query)
+    want = np.array([orc.corr(qrev, A, B, 4, s.encode(), [0], [len(s)], 2048,
+                              min(2048, orc.nextpow2(max(300, len(s), 16))))[0] for s in seqs])
+    want = want[order]
+    assert corr_tol(got, want).all(), np.max(np.abs(got - want))
+
+
 @pytest.mark.parametrize("Lq,k", [(50, 4), (200, 4), (500, 4), (200, 1), (90, 3)])
 def test_db_prefilter_any_index_order(orc, dev, Lq, k):
     """Resident-database prefilter on an index that is length-sorted but not
