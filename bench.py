@@ -103,6 +103,60 @@ def copy_probe(nbytes: int, reps: int) -> None:
     print(json.dumps({"copy_probe_bytes": nbytes, "reps": reps}), flush=True)
 
 
+def kernel_rates(db, dev, reps: int = 5) -> dict:
+    """HBM rates of the stream kernels on the first 1 GB of the packed database:
+    unpack (decompress_launcher), pack (compress_launcher) and the delimiter
+    index (sseqs_db_index), timed with HIP events on the launch stream."""
+    import torch
+
+    from sseqs_amd import _lib
+    from sseqs_amd._lib import call, ptr, stream_of
+    st = stream_of(dev)
+    nb = min(1_000_000_000, db.nbytes // 5 * 5)
+    n_sym = nb * 8 // 5
+    ascii = torch.empty(n_sym, dtype=torch.uint8, device=dev)
+    packed2 = torch.empty(nb + 64, dtype=torch.uint8, device=dev)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    t_dec = timed(lambda: call("decompress_launcher", ptr(db.packed), ptr(ascii), n_sym, 0, st))
+    t_cmp = timed(lambda: call("compress_launcher", ptr(ascii), ptr(packed2), n_sym, st))
+    lib = _lib.load()
+    ws = _lib.workspace(lib.sseqs_db_index_ws_bytes(n_sym), dev)
+    cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+    dummy = torch.empty(1, dtype=torch.int64, device=dev)
+    call("sseqs_db_index", ptr(db.packed), n_sym, ptr(dummy), ptr(dummy), 0, ptr(cnt), ptr(ws),
+         ws.numel(), st)
+    nseq = int(cnt.item())
+    off = torch.empty(nseq, dtype=torch.int64, device=dev)
+    ln = torch.empty(nseq, dtype=torch.int32, device=dev)
+    t_idx = timed(lambda: call("sseqs_db_index", ptr(db.packed), n_sym, ptr(off), ptr(ln), nseq,
+                               ptr(cnt), ptr(ws), ws.numel(), st))
+    gbs = lambda b, ms: round(b / ms / 1e6, 1)
+    return {
+        "packed_bytes": nb,
+        "decompress_ms": round(t_dec, 3),
+        "decompress_hbm_gb_per_s": gbs(nb + n_sym, t_dec),   # 5/8 B read + 1 B written per symbol
+        "compress_ms": round(t_cmp, 3),
+        "compress_hbm_gb_per_s": gbs(n_sym + nb, t_cmp),
+        "index_ms": round(t_idx, 3),
+        "index_sequences": nseq,
+        # the count and write passes each read the stream; 12 B per sequence written
+        "index_hbm_gb_per_s": gbs(2 * nb + 12 * nseq, t_idx),
+        "hbm_peak_gb_per_s": HBM_PEAK_GBPS,
+    }
+
+
 def load_traffic(kernel: str, workload: str):
     """HBM bytes per launch of `kernel` measured with rocprofv3 --pmc and
     committed under profiles/ (None when no measurement is there)."""
@@ -291,6 +345,8 @@ def main() -> int:
         return 0
     r200, launches200 = measure(q200)
     rbatch = measure_batch(q200) if args.batch_steps > 0 else None
+    rates = kernel_rates(db, dev)
+    log(f"stream kernels {json.dumps(rates)}")
     r500 = r500l = None
     if not args.no_q500:
         r500, _ = measure(q500)
@@ -349,6 +405,7 @@ def main() -> int:
                          "kernel": dom["kernel"],
                          "algorithmic": "packed bytes 5/8*(L+1) per sequence of the launch"},
         "msa_batch_cascade": rbatch,
+        "stream_kernels": rates,
         "q500": r500,
         "q500_linear4": r500l,
         "cpu_baseline": cpu,

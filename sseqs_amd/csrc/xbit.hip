@@ -54,41 +54,153 @@ __global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ in,
     }
 }
 
+// Wide forms for aligned streams: 32 symbols (four groups, 20 packed bytes)
+// per thread and step, moved as two 16-byte and five 4-byte accesses.  They
+// cover the first n/32 whole chunks; the kernels above take the tail.
+__global__ __launch_bounds__(256) void k_unpack32(const uint32_t* __restrict__ in,
+                                                  uint4* __restrict__ out, int64_t chunks) {
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < chunks;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t d[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) d[i] = __builtin_bswap32(in[5 * c + i]);
+        uint32_t o[8];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            // group g: bits 40g .. 40g+39 of the 160-bit big-endian chunk
+            const int b = 40 * g, i = b >> 5, sh = b & 31;
+            const uint64_t hi = ((uint64_t)d[i] << 32) | (i + 1 < 5 ? d[i + 1] : 0u);
+            const uint64_t w = (hi << sh) >> 24;  // the group in bits 39..0
+            uint64_t v = 0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v |= (((w >> (35 - 5 * k)) & 31u) + 64u) << (8 * k);
+            o[2 * g] = (uint32_t)v;
+            o[2 * g + 1] = (uint32_t)(v >> 32);
+        }
+        out[2 * c] = make_uint4(o[0], o[1], o[2], o[3]);
+        out[2 * c + 1] = make_uint4(o[4], o[5], o[6], o[7]);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_pack32(const uint4* __restrict__ in,
+                                                uint32_t* __restrict__ out, int64_t chunks) {
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < chunks;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        const uint4 a = in[2 * c], b = in[2 * c + 1];
+        const uint32_t x[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+        uint32_t d[5] = {0u, 0u, 0u, 0u, 0u};  // big-endian chunk
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            uint64_t w = 0;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const uint32_t byte = (x[2 * g + (k >> 2)] >> (8 * (k & 3))) & 255u;
+                w |= (uint64_t)((byte - 64u) & 31u) << (35 - 5 * k);
+            }
+            const int bit = 40 * g, i = bit >> 5, sh = bit & 31;
+            // place the 40 bits at chunk bits bit..bit+39 (from the top)
+            const uint64_t top = w << 24;  // group in bits 63..24
+            // (sh <= 24, so the 40 bits stay inside words i and i + 1)
+            d[i] |= (uint32_t)((top >> 32) >> sh);
+            d[i + 1] |= (uint32_t)((top << (32 - sh)) >> 32);
+        }
+#pragma unroll
+        for (int i = 0; i < 5; ++i) out[5 * c + i] = __builtin_bswap32(d[i]);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Database index: sequences are the maximal non-'@' runs of the stream.
-// Pass 1 counts run starts per tile, pass 2 scans tile counts (one block),
-// pass 3 writes (offset, length) for every run that starts in the tile.
+// The stream is cut into tiles of IDX_TILE symbols, 16 per thread.
+//   count: per tile, the number of runs that start in it and the position of
+//          its first '@' (-1 for none);
+//   scan:  exclusive scan of the tile counts (three launches);
+//   write: (offset, length) of every run that starts in the tile.  A run's
+//          end is found from the same bit masks when it lies in the tile;
+//          the one run that may leave the tile looks its end up in the
+//          first-'@' table of the tiles that follow.
 // A run starts at q when sym(q) != '@' and (q == 0 or sym(q-1) == '@').
 // ---------------------------------------------------------------------------
-constexpr int IDX_TILE = 4096;  // symbols per block in passes 1 and 3
+constexpr int IDX_TILE = 4096;   // symbols per tile: 256 threads x 16
+constexpr int SCAN_PER = 8192;   // tile counts per block of the scan: 1024 x 8
 
-__device__ inline bool run_start(const uint8_t* p, int64_t q) {
-    if (packed_sym(p, q) == 64u) return false;
-    return q == 0 || packed_sym(p, q - 1) == 64u;
+// Bit j of the result: symbol q0 + j (q0 a multiple of 16) is a residue,
+// i.e. not '@' and below n.  The 16 symbols are the 10 bytes at 5*q0/8, an
+// even offset, read as three aligned dwords (within the stream's tail pad).
+__device__ inline uint32_t residue_mask16(const uint8_t* __restrict__ p, int64_t n, int64_t q0) {
+    if (q0 >= n) return 0u;
+    const int64_t b0 = (q0 >> 3) * 5;
+    const uint32_t* d = reinterpret_cast<const uint32_t*>(p + (b0 & ~(int64_t)3));
+    const uint64_t w0 = __builtin_bswap32(d[0]), w1 = __builtin_bswap32(d[1]),
+                   w2 = __builtin_bswap32(d[2]);
+    const uint64_t X = (w0 << 32) | w1, Y = (w1 << 32) | w2;
+    const bool odd = (b0 & 3) != 0;  // 10-byte window starts 2 bytes into the first dword
+    const uint64_t M40 = (1ull << 40) - 1;
+    uint64_t g[2] = {(odd ? X >> 8 : X >> 24) & M40, (odd ? Y : Y >> 16) & M40};
+    uint32_t m = 0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        // OR of each 5-bit field lands on the field's lowest bit
+        const uint64_t f = g[h] | (g[h] >> 1) | (g[h] >> 2) | (g[h] >> 3) | (g[h] >> 4);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) m |= (uint32_t)((f >> (35 - 5 * k)) & 1u) << (8 * h + k);
+    }
+    if (q0 + 16 > n) m &= (1u << (uint32_t)(n - q0)) - 1u;
+    return m;
+}
+
+__device__ inline bool residue_at(const uint8_t* p, int64_t n, int64_t q) {
+    return q >= 0 && q < n && packed_sym(p, q) != 64u;
+}
+
+__device__ inline int wave_sum_i32(int v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ inline int wave_min_i32(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
 }
 
 __global__ __launch_bounds__(256) void k_idx_count(const uint8_t* __restrict__ p, int64_t n,
-                                                   int64_t nt, int64_t* __restrict__ tile_cnt) {
-    __shared__ int red[256];
+                                                   int64_t nt, int64_t* __restrict__ tile_cnt,
+                                                   int64_t* __restrict__ first_at) {
+    __shared__ uint32_t mk[256];
+    __shared__ int red[2][4];
+    __shared__ int prev_res;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     for (int64_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
         const int64_t base = tile * IDX_TILE;
-        int c = 0;
-        for (int i = threadIdx.x; i < IDX_TILE; i += 256) {
-            int64_t q = base + i;
-            if (q < n && run_start(p, q)) ++c;
-        }
-        red[threadIdx.x] = c;
+        const uint32_t m = residue_mask16(p, n, base + 16 * tid);
+        mk[tid] = m;
+        if (tid == 0) prev_res = residue_at(p, n, base - 1);
         __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
-            __syncthreads();
+        const uint32_t prev = tid ? (mk[tid - 1] >> 15) & 1u : (uint32_t)prev_res;
+        const uint32_t starts = m & ~((m << 1) | prev) & 0xFFFFu;
+        // first '@' below n: a zero bit of m at a position < n
+        int64_t lim = n - (base + 16 * tid);
+        uint32_t valid = lim >= 16 ? 0xFFFFu : lim <= 0 ? 0u : (1u << (uint32_t)lim) - 1u;
+        const uint32_t at = ~m & valid;
+        int c = wave_sum_i32(__popc(starts));
+        int z = wave_min_i32(at ? 16 * tid + __builtin_ctz(at) : IDX_TILE);
+        if (lane == 0) {
+            red[0][wid] = c;
+            red[1][wid] = z;
         }
-        if (threadIdx.x == 0) tile_cnt[tile] = red[0];
+        __syncthreads();
+        if (tid == 0) {
+            tile_cnt[tile] = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+            const int zz = min(min(red[1][0], red[1][1]), min(red[1][2], red[1][3]));
+            first_at[tile] = zz < IDX_TILE ? base + zz : -1;
+        }
         __syncthreads();
     }
 }
 
-// Exclusive scan of ntile counts in one block of 1024 threads; total -> *total.
+// Scan of the tile counts: per-block sums, a one-block exclusive scan of the
+// sums (k_scan_i64), then each block's exclusive scan with its offset.
+// Exclusive scan of nt values in one block of 1024 threads; total -> *total.
 __global__ __launch_bounds__(1024) void k_scan_i64(int64_t* __restrict__ v, int64_t nt,
                                                    int64_t* __restrict__ total) {
     __shared__ int64_t part[1024];
@@ -113,37 +225,128 @@ __global__ __launch_bounds__(1024) void k_scan_i64(int64_t* __restrict__ v, int6
     if (threadIdx.x == 1023) *total = part[1023];
 }
 
-__global__ __launch_bounds__(256) void k_idx_write(const uint8_t* __restrict__ p, int64_t n,
-                                                   int64_t nt, const int64_t* __restrict__ tile_base,
-                                                   int64_t* __restrict__ off,
-                                                   int32_t* __restrict__ len, int64_t cap) {
-    __shared__ int wsum[4];
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    for (int64_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
-    const int64_t base = tile * IDX_TILE;
-    int64_t out = tile_base[tile];
-    for (int i0 = 0; i0 < IDX_TILE; i0 += 256) {
-        int64_t q = base + i0 + threadIdx.x;
-        bool st = q < n && run_start(p, q);
-        uint64_t m = __ballot(st);
-        int before = __popcll(m & ((1ull << lane) - 1ull));
-        if (lane == 0) wsum[wid] = __popcll(m);
+__global__ __launch_bounds__(1024) void k_scan_up(const int64_t* __restrict__ v, int64_t nt,
+                                                  int64_t* __restrict__ part) {
+    __shared__ int64_t red[16];
+    const int tid = threadIdx.x;
+    for (int64_t b = blockIdx.x; b * SCAN_PER < nt; b += gridDim.x) {
+        const int64_t lo = b * SCAN_PER + 8 * (int64_t)tid;
+        int64_t s = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) s += lo + k < nt ? v[lo + k] : 0;
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if ((tid & 63) == 0) red[tid >> 6] = s;
         __syncthreads();
-        int wb = 0;
-        for (int w = 0; w < wid; ++w) wb += wsum[w];
-        int tot = wsum[0] + wsum[1] + wsum[2] + wsum[3];
-        if (st) {
-            int64_t slot = out + wb + before;
-            if (slot < cap) {
-                int64_t e = q;
-                while (e < n && packed_sym(p, e) != 64u) ++e;
-                off[slot] = q;
-                len[slot] = (int32_t)(e - q);
-            }
+        if (tid == 0) {
+            int64_t t = 0;
+            for (int w = 0; w < 16; ++w) t += red[w];
+            part[b] = t;
         }
-        out += tot;
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(1024) void k_scan_down(int64_t* __restrict__ v, int64_t nt,
+                                                    const int64_t* __restrict__ part) {
+    __shared__ int64_t wsum[16];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    for (int64_t b = blockIdx.x; b * SCAN_PER < nt; b += gridDim.x) {
+        const int64_t lo = b * SCAN_PER + 8 * (int64_t)tid;
+        int64_t x[8], s = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            x[k] = lo + k < nt ? v[lo + k] : 0;
+            s += x[k];
+        }
+        int64_t inc = s;  // inclusive scan over the wave
+        for (int off = 1; off < 64; off <<= 1) {
+            int64_t t = __shfl_up(inc, off, 64);
+            if (lane >= off) inc += t;
+        }
+        if (lane == 63) wsum[wid] = inc;
+        __syncthreads();
+        int64_t run = part[b] + inc - s;
+        for (int w = 0; w < wid; ++w) run += wsum[w];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (lo + k < nt) v[lo + k] = run;
+            run += x[k];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_idx_write(const uint8_t* __restrict__ p, int64_t n,
+                                                   int64_t nt, const int64_t* __restrict__ tile_base,
+                                                   const int64_t* __restrict__ first_at,
+                                                   int64_t* __restrict__ off,
+                                                   int32_t* __restrict__ len, int64_t cap) {
+    __shared__ uint32_t mk[256];
+    __shared__ uint16_t st[IDX_TILE / 2];  // local start of the tile's j-th run
+    __shared__ int wsum[4];
+    __shared__ int edge[2];                // residue just before / just after the tile
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    for (int64_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
+        const int64_t base = tile * IDX_TILE;
+        const int64_t out = tile_base[tile];
+        const uint32_t m = residue_mask16(p, n, base + 16 * tid);
+        mk[tid] = m;
+        if (tid == 0) {
+            edge[0] = residue_at(p, n, base - 1);
+            edge[1] = residue_at(p, n, base + IDX_TILE);
+        }
+        __syncthreads();
+        const uint32_t prev = tid ? (mk[tid - 1] >> 15) & 1u : (uint32_t)edge[0];
+        const uint32_t next = tid < 255 ? mk[tid + 1] & 1u : (uint32_t)edge[1];
+        const uint32_t starts = m & ~((m << 1) | prev) & 0xFFFFu;
+        const uint32_t ends = m & ~((m >> 1) | (next << 15));
+        const int c = __popc(starts);
+        int inc = c;  // inclusive scan of start counts over the wave
+        for (int o = 1; o < 64; o <<= 1) {
+            int t = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += t;
+        }
+        if (lane == 63) wsum[wid] = inc;
+        __syncthreads();
+        int ex = inc - c;
+        for (int w = 0; w < wid; ++w) ex += wsum[w];
+        const int total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
+        {
+            uint32_t s = starts;
+            int j = ex;
+            while (s) {
+                const int b = __builtin_ctz(s);
+                s &= s - 1;
+                st[j] = (uint16_t)(16 * tid + b);
+                if (out + j < cap) off[out + j] = base + 16 * tid + b;
+                ++j;
+            }
+        }
+        __syncthreads();
+        {
+            // an end at local e closes the latest run started at or before e;
+            // none started in the tile means the run began in an earlier tile,
+            // whose block writes its length
+            uint32_t e = ends;
+            while (e) {
+                const int b = __builtin_ctz(e);
+                e &= e - 1;
+                const int j = ex + __popc(starts & ((2u << b) - 1u)) - 1;
+                if (j >= 0 && out + j < cap) len[out + j] = 16 * tid + b + 1 - st[j];
+            }
+        }
+        if (tid == 255 && total > 0 && ((m >> 15) & 1u) && next) {
+            // the tile's last run continues past it: it ends at the first '@'
+            // of a later tile, or at the end of the stream
+            const int j = total - 1;
+            if (out + j < cap) {
+                int64_t t = tile + 1;
+                while (t < nt && first_at[t] < 0) ++t;
+                const int64_t e = t < nt ? first_at[t] : n;
+                len[out + j] = (int32_t)(e - (base + st[j]));
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -184,8 +387,20 @@ extern "C" int decompress_launcher(const uint8_t* d_in, uint8_t* d_out, int64_t 
     if (n_syms < 0 || lo < 0) return fail(-1, "decompress: negative size or offset");
     if (n_syms == 0) return 0;
     if (!d_in || !d_out) return fail(-1, "decompress: null pointer");
-    hipLaunchKernelGGL(k_unpack, dim3(grid_cap(ceil_div(n_syms, 8), 256)), dim3(256), 0, stream,
-                       d_in + lo, d_out, n_syms);
+    const uint8_t* src = d_in + lo;
+    int64_t chunks = 0;
+    if ((reinterpret_cast<uintptr_t>(src) & 3) == 0 && (reinterpret_cast<uintptr_t>(d_out) & 15) == 0)
+        chunks = n_syms / 32;
+    if (chunks) {
+        hipLaunchKernelGGL(k_unpack32, dim3(grid_cap(chunks, 256)), dim3(256), 0, stream,
+                           reinterpret_cast<const uint32_t*>(src), reinterpret_cast<uint4*>(d_out),
+                           chunks);
+        if (int r = check_launch("decompress")) return r;
+    }
+    const int64_t rest = n_syms - 32 * chunks;
+    if (rest == 0) return 0;
+    hipLaunchKernelGGL(k_unpack, dim3(grid_cap(ceil_div(rest, 8), 256)), dim3(256), 0, stream,
+                       src + 20 * chunks, d_out + 32 * chunks, rest);
     return check_launch("decompress");
 }
 
@@ -194,28 +409,50 @@ extern "C" int compress_launcher(const uint8_t* d_in, uint8_t* d_out, int64_t n_
     if (n_syms < 0) return fail(-1, "compress: negative size");
     if (n_syms == 0) return 0;
     if (!d_in || !d_out) return fail(-1, "compress: null pointer");
-    hipLaunchKernelGGL(k_pack, dim3(grid_cap(ceil_div(n_syms, 8), 256)), dim3(256), 0, stream, d_in,
-                       d_out, n_syms);
+    int64_t chunks = 0;
+    if ((reinterpret_cast<uintptr_t>(d_in) & 15) == 0 && (reinterpret_cast<uintptr_t>(d_out) & 3) == 0)
+        chunks = n_syms / 32;
+    if (chunks) {
+        hipLaunchKernelGGL(k_pack32, dim3(grid_cap(chunks, 256)), dim3(256), 0, stream,
+                           reinterpret_cast<const uint4*>(d_in), reinterpret_cast<uint32_t*>(d_out),
+                           chunks);
+        if (int r = check_launch("compress")) return r;
+    }
+    const int64_t rest = n_syms - 32 * chunks;
+    if (rest == 0) return 0;
+    hipLaunchKernelGGL(k_pack, dim3(grid_cap(ceil_div(rest, 8), 256)), dim3(256), 0, stream,
+                       d_in + 32 * chunks, d_out + 20 * chunks, rest);
     return check_launch("compress");
 }
 
 extern "C" size_t sseqs_db_index_ws_bytes(int64_t n_syms) {
-    return (size_t)(ceil_div(n_syms, IDX_TILE) + 1) * sizeof(int64_t);
+    const int64_t nt = ceil_div(n_syms > 0 ? n_syms : 1, IDX_TILE);
+    return (size_t)(2 * nt + ceil_div(nt, SCAN_PER) + 1) * sizeof(int64_t);
 }
 
 extern "C" int sseqs_db_index(const uint8_t* packed, int64_t n_syms, int64_t* seq_off,
                               int32_t* seq_len, int64_t cap, int64_t* d_nseq, void* ws,
                               size_t ws_bytes, hipStream_t stream) {
     if (n_syms <= 0) return fail(-1, "db_index: empty database");
+    if (!packed || !d_nseq || !ws || (cap > 0 && (!seq_off || !seq_len)))
+        return fail(-1, "db_index: null pointer");
+    if (reinterpret_cast<uintptr_t>(packed) & 3) return fail(-1, "db_index: packed must be 4-byte aligned");
     if (ws_bytes < sseqs_db_index_ws_bytes(n_syms)) return fail(-2, "db_index: workspace too small");
-    int64_t nt = ceil_div(n_syms, IDX_TILE);
+    const int64_t nt = ceil_div(n_syms, IDX_TILE), nb = ceil_div(nt, SCAN_PER);
     int64_t* tiles = static_cast<int64_t*>(ws);
+    int64_t* first_at = tiles + nt;
+    int64_t* part = first_at + nt;
     hipLaunchKernelGGL(k_idx_count, dim3(grid_cap(nt, 1)), dim3(256), 0, stream, packed, n_syms, nt,
-                       tiles);
+                       tiles, first_at);
     if (int r = check_launch("db_index count")) return r;
-    hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, stream, tiles, nt, d_nseq);
+    hipLaunchKernelGGL(k_scan_up, dim3(grid_cap(nb, 1)), dim3(1024), 0, stream, tiles, nt, part);
     if (int r = check_launch("db_index scan")) return r;
+    hipLaunchKernelGGL(k_scan_i64, dim3(1), dim3(1024), 0, stream, part, nb, d_nseq);
+    if (int r = check_launch("db_index scan")) return r;
+    hipLaunchKernelGGL(k_scan_down, dim3(grid_cap(nb, 1)), dim3(1024), 0, stream, tiles, nt, part);
+    if (int r = check_launch("db_index scan")) return r;
+    if (cap <= 0) return 0;
     hipLaunchKernelGGL(k_idx_write, dim3(grid_cap(nt, 1)), dim3(256), 0, stream, packed, n_syms, nt,
-                       tiles, seq_off, seq_len, cap);
+                       tiles, first_at, seq_off, seq_len, cap);
     return check_launch("db_index write");
 }

@@ -66,6 +66,14 @@ def test_argument_errors_need_no_gpu():
     assert lib.sseqs_db_sw_linear(None, None, None, None, 5, None, 10, 2, None, None) == -1
     assert lib.sseqs_db_sw_affine(None, None, None, None, 5, None, 2049, 11, 1, None, None) == -1
     assert lib.sseqs_db_sw_affine(None, None, None, None, -1, None, 20, 11, 1, None, None) == -1
+    assert lib.sseqs_db_index(None, 0, None, None, 0, None, None, 0, None) == -1
+    assert lib.sseqs_db_index(None, 100, None, None, 0, None, None, 0, None) == -1
+    assert b"null" in lib.sseqs_last_error()
+    # a misaligned stream is refused before anything is launched
+    assert lib.sseqs_db_index(3, 100, None, None, 0, 8, 8, 1 << 20, None) == -1
+    assert b"aligned" in lib.sseqs_last_error()
+    assert lib.sseqs_db_index(64, 100, None, None, 0, 8, 8, 8, None) == -2
+    assert lib.sseqs_db_index_ws_bytes(1 << 40) > lib.sseqs_db_index_ws_bytes(1 << 20)
 
 
 def test_prefilter_kernel_plan_needs_no_gpu():

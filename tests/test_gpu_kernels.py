@@ -42,7 +42,7 @@ def corr_tol(got, want):
 
 # ----------------------------------------------------------------- codec --
 
-@pytest.mark.parametrize("n", [1, 7, 8, 9, 1000, 123457])
+@pytest.mark.parametrize("n", [1, 7, 8, 9, 31, 32, 33, 69, 1000, 123457])
 def test_xbit_roundtrip(orc, dev, n):
     from sseqs_amd import ops
     announce(f"xbit n={n}")
@@ -51,10 +51,27 @@ def test_xbit_roundtrip(orc, dev, n):
     packed = ops.compress(torch.from_numpy(s).to(dev))
     assert np.array_equal(packed.cpu().numpy(), orc.xbit_pack(s))
     assert np.array_equal(ops.decompress(packed, n, 0).cpu().numpy(), s)
-    for g in (1, 3):
+    # byte offsets 5 and 15 are not 4-aligned (byte-wise kernel), 20 and 40 are
+    for g in (1, 3, 4, 8):
         if 8 * g < n:
             out = ops.decompress(packed, n - 8 * g, 5 * g).cpu().numpy()
             assert np.array_equal(out, s[8 * g:])
+
+
+@pytest.mark.parametrize("n", [40, 4099, 70001])
+def test_xbit_any_bytes_and_alignment(orc, dev, n):
+    """Bytes outside '@'..'_' keep their low 5 bits of (b - 64), as in
+    sseqs/xbit.cu:36-62; inputs at odd addresses take the byte-wise kernel."""
+    from sseqs_amd import ops
+    announce(f"xbit any bytes n={n}")
+    rng = np.random.default_rng(n + 1)
+    s = rng.integers(0, 256, n + 3).astype(np.uint8)
+    t = torch.from_numpy(s).to(dev)
+    for shift in (0, 1, 3):
+        packed = ops.compress(t[shift:shift + n])
+        assert np.array_equal(packed.cpu().numpy(), orc.xbit_pack(s[shift:shift + n])), shift
+    back = ops.decompress(ops.compress(t[:n]), n, 0).cpu().numpy()
+    assert np.array_equal(back, ((s[:n].astype(np.int32) - 64) & 31) + 64)
 
 
 def test_xbit_host_input_moves_to_gpu(dev):
@@ -112,6 +129,58 @@ def test_db_index_and_gather(orc, dev):
     got = db.strings(ids, 500)
     want = [seqs[order[i]][:500] for i in ids.cpu().numpy()]
     assert got == want
+
+
+def _index_streams(rng):
+    """Residue (1) / '@' (0) patterns around the index's 4096-symbol tiles:
+    runs spanning several tiles, runs that end or start on a tile edge, a
+    stream that is one run, alternation (2048 runs in a tile) and random
+    densities."""
+    T = 4096
+    out = []
+    one = np.ones(3 * T + 5, bool)
+    out.append(one.copy())
+    for cut in ([T - 1], [T], [T + 1], [T - 1, T], [2 * T - 1, 2 * T], [5], [3 * T + 4]):
+        s = one.copy()
+        s[cut] = False
+        out.append(s)
+    alt = np.zeros(2 * T + 3, bool)
+    alt[::2] = True
+    out.append(alt)
+    out.append(~alt)
+    for n, dens in ((T, 0.5), (T + 16, 0.02), (7 * T + 9, 0.003), (40 * T + 1, 0.0002), (1, 0.0)):
+        out.append(rng.random(n) >= dens)
+    for n in (1, 15, 16, 17, T - 1, T, T + 1):
+        out.append(np.ones(n, bool))
+    return out
+
+
+def test_db_index_tile_edges(dev):
+    from sseqs_amd import _lib
+    from sseqs_amd._lib import call, ptr, stream_of
+    announce("db_index tile edges")
+    rng = np.random.default_rng(11)
+    lib = _lib.load()
+    for k, res in enumerate(_index_streams(rng)):
+        n = res.size
+        sym = np.where(res, rng.integers(65, 91, n), 64).astype(np.uint8)
+        packed = torch.zeros((n * 5 + 7) // 8 + 64, dtype=torch.uint8, device=dev)
+        c = _packed_db(dev, sym.tobytes())
+        packed[: c.numel()] = c
+        d = np.diff(np.concatenate([[0], res.astype(np.int8), [0]]))
+        want_off = np.nonzero(d == 1)[0]
+        want_len = np.nonzero(d == -1)[0] - want_off
+        ws = _lib.workspace(lib.sseqs_db_index_ws_bytes(n), dev)
+        cnt = torch.zeros(1, dtype=torch.int64, device=dev)
+        cap = max(1, want_off.size)
+        off = torch.full((cap,), -1, dtype=torch.int64, device=dev)
+        ln = torch.full((cap,), -1, dtype=torch.int32, device=dev)
+        call("sseqs_db_index", ptr(packed), n, ptr(off), ptr(ln), cap, ptr(cnt), ptr(ws),
+             ws.numel(), stream_of(dev))
+        assert int(cnt.item()) == want_off.size, k
+        m = want_off.size
+        assert np.array_equal(off.cpu().numpy()[:m], want_off), k
+        assert np.array_equal(ln.cpu().numpy()[:m], want_len), k
 
 
 # -------------------------------------------------------------------- SW --
