@@ -143,6 +143,11 @@ def kernel_rates(db, dev, reps: int = 5) -> dict:
     t_idx = timed(lambda: call("sseqs_db_index", ptr(db.packed), n_sym, ptr(off), ptr(ln), nseq,
                                ptr(cnt), ptr(ws), ws.numel(), st))
     gbs = lambda b, ms: round(b / ms / 1e6, 1)
+    # a plain device copy of the unpacked size: what a kernel that reads and
+    # writes HBM reaches on this GPU
+    copy_dst = torch.empty_like(ascii)
+    t_copy = timed(lambda: copy_dst.copy_(ascii))
+    del copy_dst
     return {
         "packed_bytes": nb,
         "decompress_ms": round(t_dec, 3),
@@ -153,6 +158,7 @@ def kernel_rates(db, dev, reps: int = 5) -> dict:
         "index_sequences": nseq,
         # the count and write passes each read the stream; 12 B per sequence written
         "index_hbm_gb_per_s": gbs(2 * nb + 12 * nseq, t_idx),
+        "device_copy_gb_per_s": gbs(2 * n_sym, t_copy),
         "hbm_peak_gb_per_s": HBM_PEAK_GBPS,
     }
 
@@ -185,13 +191,15 @@ def main() -> int:
                     help="run only the per-batch cascade measurement (for traces)")
     ap.add_argument("--prefilter-only", action="store_true",
                     help="run only the q_len=200 prefilter sweep --steps times (for counters)")
+    ap.add_argument("--stream-only", action="store_true",
+                    help="time only the unpack, pack and index kernels (for traces)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="run a plain device copy of BYTES bytes and exit (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the database is replicated per GPU and queries are independent, "
-              "so there is no multi-GPU path to measure)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(sharding the database over several GPUs is designed but not implemented)",
+              file=sys.stderr)
         return 1
     if args.copy_probe:
         copy_probe(args.copy_probe, max(1, args.steps))
@@ -214,6 +222,9 @@ def main() -> int:
     torch.cuda.synchronize()
     log(f"generated {db.n} sequences, {db.residues} residues, {db.nbytes / 1e9:.2f} GB packed "
         f"in {time.time() - t0:.1f} s")
+    if args.stream_only:
+        print(json.dumps({"stream_kernels": kernel_rates(db, dev)}), flush=True)
+        return 0
     scores_buf = torch.empty(db.n, dtype=torch.float32, device=dev)
 
     if args.prefilter_only:

@@ -55,63 +55,92 @@ __global__ __launch_bounds__(256) void k_pack(const uint8_t* __restrict__ in,
 }
 
 // Wide forms for aligned streams: 32 symbols (four groups, 20 packed bytes)
-// per thread and step, moved as two 16-byte and five 4-byte accesses.  They
-// cover the first n/32 whole chunks; the kernels above take the tail.
+// per chunk, moved as two 16-byte and five 4-byte accesses.  Each thread
+// loads WIDE_U chunks (at a grid stride, clamped into range) before it
+// converts and stores any, so that enough bytes are in flight per wave.
+// They cover the first n/32 whole chunks; the kernels above take the tail.
+constexpr int WIDE_U = 4;
+
 __global__ __launch_bounds__(256) void k_unpack32(const uint32_t* __restrict__ in,
                                                   uint4* __restrict__ out, int64_t chunks) {
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < chunks;
-         c += (int64_t)gridDim.x * blockDim.x) {
-        uint32_t d[5];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t c0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c0 < chunks;
+         c0 += WIDE_U * stride) {
+        uint32_t d[WIDE_U][5];
 #pragma unroll
-        for (int i = 0; i < 5; ++i) d[i] = __builtin_bswap32(in[5 * c + i]);
-        uint32_t o[8];
+        for (int u = 0; u < WIDE_U; ++u) {
+            const int64_t c = min(c0 + u * stride, chunks - 1);
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            // group g: bits 40g .. 40g+39 of the 160-bit big-endian chunk
-            const int b = 40 * g, i = b >> 5, sh = b & 31;
-            const uint64_t hi = ((uint64_t)d[i] << 32) | (i + 1 < 5 ? d[i + 1] : 0u);
-            const uint64_t w = (hi << sh) >> 24;  // the group in bits 39..0
-            uint64_t v = 0;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) v |= (((w >> (35 - 5 * k)) & 31u) + 64u) << (8 * k);
-            o[2 * g] = (uint32_t)v;
-            o[2 * g + 1] = (uint32_t)(v >> 32);
+            for (int i = 0; i < 5; ++i) d[u][i] = __builtin_bswap32(in[5 * c + i]);
         }
-        out[2 * c] = make_uint4(o[0], o[1], o[2], o[3]);
-        out[2 * c + 1] = make_uint4(o[4], o[5], o[6], o[7]);
+#pragma unroll
+        for (int u = 0; u < WIDE_U; ++u) {
+            const int64_t c = c0 + u * stride;
+            uint32_t o[8];
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                // group g: bits 40g .. 40g+39 of the 160-bit big-endian chunk
+                const int b = 40 * g, i = b >> 5, sh = b & 31;
+                const uint64_t hi = ((uint64_t)d[u][i] << 32) | d[u][i + 1];
+                const uint64_t w = (hi << sh) >> 24;  // the group in bits 39..0
+                uint64_t v = 0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) v |= (((w >> (35 - 5 * k)) & 31u) + 64u) << (8 * k);
+                o[2 * g] = (uint32_t)v;
+                o[2 * g + 1] = (uint32_t)(v >> 32);
+            }
+            if (c < chunks) {
+                out[2 * c] = make_uint4(o[0], o[1], o[2], o[3]);
+                out[2 * c + 1] = make_uint4(o[4], o[5], o[6], o[7]);
+            }
+        }
     }
 }
 
 __global__ __launch_bounds__(256) void k_pack32(const uint4* __restrict__ in,
                                                 uint32_t* __restrict__ out, int64_t chunks) {
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < chunks;
-         c += (int64_t)gridDim.x * blockDim.x) {
-        const uint4 a = in[2 * c], b = in[2 * c + 1];
-        const uint32_t x[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-        uint32_t d[5] = {0u, 0u, 0u, 0u, 0u};  // big-endian chunk
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t c0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c0 < chunks;
+         c0 += WIDE_U * stride) {
+        uint4 a[WIDE_U][2];
 #pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            uint64_t w = 0;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const uint32_t byte = (x[2 * g + (k >> 2)] >> (8 * (k & 3))) & 255u;
-                w |= (uint64_t)((byte - 64u) & 31u) << (35 - 5 * k);
-            }
-            const int bit = 40 * g, i = bit >> 5, sh = bit & 31;
-            // place the 40 bits at chunk bits bit..bit+39 (from the top)
-            const uint64_t top = w << 24;  // group in bits 63..24
-            // (sh <= 24, so the 40 bits stay inside words i and i + 1)
-            d[i] |= (uint32_t)((top >> 32) >> sh);
-            d[i + 1] |= (uint32_t)((top << (32 - sh)) >> 32);
+        for (int u = 0; u < WIDE_U; ++u) {
+            const int64_t c = min(c0 + u * stride, chunks - 1);
+            a[u][0] = in[2 * c];
+            a[u][1] = in[2 * c + 1];
         }
 #pragma unroll
-        for (int i = 0; i < 5; ++i) out[5 * c + i] = __builtin_bswap32(d[i]);
+        for (int u = 0; u < WIDE_U; ++u) {
+            const int64_t c = c0 + u * stride;
+            const uint32_t x[8] = {a[u][0].x, a[u][0].y, a[u][0].z, a[u][0].w,
+                                   a[u][1].x, a[u][1].y, a[u][1].z, a[u][1].w};
+            uint32_t d[5] = {0u, 0u, 0u, 0u, 0u};  // big-endian chunk
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                uint64_t w = 0;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    const uint32_t byte = (x[2 * g + (k >> 2)] >> (8 * (k & 3))) & 255u;
+                    w |= (uint64_t)((byte - 64u) & 31u) << (35 - 5 * k);
+                }
+                const int bit = 40 * g, i = bit >> 5, sh = bit & 31;
+                // place the 40 bits at chunk bits bit..bit+39 (from the top);
+                // sh <= 24, so they stay inside words i and i + 1
+                const uint64_t top = w << 24;  // group in bits 63..24
+                d[i] |= (uint32_t)((top >> 32) >> sh);
+                d[i + 1] |= (uint32_t)((top << (32 - sh)) >> 32);
+            }
+            if (c < chunks) {
+#pragma unroll
+                for (int i = 0; i < 5; ++i) out[5 * c + i] = __builtin_bswap32(d[i]);
+            }
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
 // Database index: sequences are the maximal non-'@' runs of the stream.
-// The stream is cut into tiles of IDX_TILE symbols, 16 per thread.
+// The stream is cut into tiles of IDX_TILE symbols, 64 per thread.
 //   count: per tile, the number of runs that start in it and the position of
 //          its first '@' (-1 for none);
 //   scan:  exclusive scan of the tile counts (three launches);
@@ -121,31 +150,30 @@ __global__ __launch_bounds__(256) void k_pack32(const uint4* __restrict__ in,
 //          first-'@' table of the tiles that follow.
 // A run starts at q when sym(q) != '@' and (q == 0 or sym(q-1) == '@').
 // ---------------------------------------------------------------------------
-constexpr int IDX_TILE = 4096;   // symbols per tile: 256 threads x 16
+constexpr int IDX_TILE = 16384;  // symbols per tile: 256 threads x 64
 constexpr int SCAN_PER = 8192;   // tile counts per block of the scan: 1024 x 8
 
-// Bit j of the result: symbol q0 + j (q0 a multiple of 16) is a residue,
-// i.e. not '@' and below n.  The 16 symbols are the 10 bytes at 5*q0/8, an
-// even offset, read as three aligned dwords (within the stream's tail pad).
-__device__ inline uint32_t residue_mask16(const uint8_t* __restrict__ p, int64_t n, int64_t q0) {
-    if (q0 >= n) return 0u;
-    const int64_t b0 = (q0 >> 3) * 5;
-    const uint32_t* d = reinterpret_cast<const uint32_t*>(p + (b0 & ~(int64_t)3));
-    const uint64_t w0 = __builtin_bswap32(d[0]), w1 = __builtin_bswap32(d[1]),
-                   w2 = __builtin_bswap32(d[2]);
-    const uint64_t X = (w0 << 32) | w1, Y = (w1 << 32) | w2;
-    const bool odd = (b0 & 3) != 0;  // 10-byte window starts 2 bytes into the first dword
-    const uint64_t M40 = (1ull << 40) - 1;
-    uint64_t g[2] = {(odd ? X >> 8 : X >> 24) & M40, (odd ? Y : Y >> 16) & M40};
-    uint32_t m = 0;
+// Bit j of the result: symbol q0 + j (q0 a multiple of 64) is a residue,
+// i.e. not '@' and below n.  The 64 symbols are the 40 bytes at 5*q0/8, read
+// as ten aligned dwords (within the stream's tail pad).
+__device__ inline uint64_t residue_mask64(const uint8_t* __restrict__ p, int64_t n, int64_t q0) {
+    if (q0 >= n) return 0ull;
+    const uint32_t* d = reinterpret_cast<const uint32_t*>(p + (q0 >> 3) * 5);
+    uint32_t D[10];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int i = 0; i < 10; ++i) D[i] = __builtin_bswap32(d[i]);
+    uint64_t m = 0;
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+        // group h: bits 40h .. 40h+39 of the 320-bit big-endian window
+        const int b = 40 * h, i = b >> 5, sh = b & 31;
+        const uint64_t w = ((((uint64_t)D[i] << 32) | D[i + 1]) << sh) >> 24;
         // OR of each 5-bit field lands on the field's lowest bit
-        const uint64_t f = g[h] | (g[h] >> 1) | (g[h] >> 2) | (g[h] >> 3) | (g[h] >> 4);
+        const uint64_t f = w | (w >> 1) | (w >> 2) | (w >> 3) | (w >> 4);
 #pragma unroll
-        for (int k = 0; k < 8; ++k) m |= (uint32_t)((f >> (35 - 5 * k)) & 1u) << (8 * h + k);
+        for (int k = 0; k < 8; ++k) m |= ((f >> (35 - 5 * k)) & 1ull) << (8 * h + k);
     }
-    if (q0 + 16 > n) m &= (1u << (uint32_t)(n - q0)) - 1u;
+    if (q0 + 64 > n) m &= (1ull << (uint32_t)(n - q0)) - 1ull;
     return m;
 }
 
@@ -166,24 +194,24 @@ __device__ inline int wave_min_i32(int v) {
 __global__ __launch_bounds__(256) void k_idx_count(const uint8_t* __restrict__ p, int64_t n,
                                                    int64_t nt, int64_t* __restrict__ tile_cnt,
                                                    int64_t* __restrict__ first_at) {
-    __shared__ uint32_t mk[256];
+    __shared__ uint64_t mk[256];
     __shared__ int red[2][4];
     __shared__ int prev_res;
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     for (int64_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
         const int64_t base = tile * IDX_TILE;
-        const uint32_t m = residue_mask16(p, n, base + 16 * tid);
+        const uint64_t m = residue_mask64(p, n, base + 64 * tid);
         mk[tid] = m;
         if (tid == 0) prev_res = residue_at(p, n, base - 1);
         __syncthreads();
-        const uint32_t prev = tid ? (mk[tid - 1] >> 15) & 1u : (uint32_t)prev_res;
-        const uint32_t starts = m & ~((m << 1) | prev) & 0xFFFFu;
+        const uint64_t prev = tid ? mk[tid - 1] >> 63 : (uint64_t)prev_res;
+        const uint64_t starts = m & ~((m << 1) | prev);
         // first '@' below n: a zero bit of m at a position < n
-        int64_t lim = n - (base + 16 * tid);
-        uint32_t valid = lim >= 16 ? 0xFFFFu : lim <= 0 ? 0u : (1u << (uint32_t)lim) - 1u;
-        const uint32_t at = ~m & valid;
-        int c = wave_sum_i32(__popc(starts));
-        int z = wave_min_i32(at ? 16 * tid + __builtin_ctz(at) : IDX_TILE);
+        const int64_t lim = n - (base + 64 * tid);
+        const uint64_t valid = lim >= 64 ? ~0ull : lim <= 0 ? 0ull : (1ull << (uint32_t)lim) - 1ull;
+        const uint64_t at = ~m & valid;
+        const int c = wave_sum_i32(__popcll(starts));
+        const int z = wave_min_i32(at ? 64 * tid + __builtin_ctzll(at) : IDX_TILE);
         if (lane == 0) {
             red[0][wid] = c;
             red[1][wid] = z;
@@ -281,7 +309,7 @@ __global__ __launch_bounds__(256) void k_idx_write(const uint8_t* __restrict__ p
                                                    const int64_t* __restrict__ first_at,
                                                    int64_t* __restrict__ off,
                                                    int32_t* __restrict__ len, int64_t cap) {
-    __shared__ uint32_t mk[256];
+    __shared__ uint64_t mk[256];
     __shared__ uint16_t st[IDX_TILE / 2];  // local start of the tile's j-th run
     __shared__ int wsum[4];
     __shared__ int edge[2];                // residue just before / just after the tile
@@ -289,18 +317,18 @@ __global__ __launch_bounds__(256) void k_idx_write(const uint8_t* __restrict__ p
     for (int64_t tile = blockIdx.x; tile < nt; tile += gridDim.x) {
         const int64_t base = tile * IDX_TILE;
         const int64_t out = tile_base[tile];
-        const uint32_t m = residue_mask16(p, n, base + 16 * tid);
+        const uint64_t m = residue_mask64(p, n, base + 64 * tid);
         mk[tid] = m;
         if (tid == 0) {
             edge[0] = residue_at(p, n, base - 1);
             edge[1] = residue_at(p, n, base + IDX_TILE);
         }
         __syncthreads();
-        const uint32_t prev = tid ? (mk[tid - 1] >> 15) & 1u : (uint32_t)edge[0];
-        const uint32_t next = tid < 255 ? mk[tid + 1] & 1u : (uint32_t)edge[1];
-        const uint32_t starts = m & ~((m << 1) | prev) & 0xFFFFu;
-        const uint32_t ends = m & ~((m >> 1) | (next << 15));
-        const int c = __popc(starts);
+        const uint64_t prev = tid ? mk[tid - 1] >> 63 : (uint64_t)edge[0];
+        const uint64_t next = tid < 255 ? mk[tid + 1] & 1ull : (uint64_t)edge[1];
+        const uint64_t starts = m & ~((m << 1) | prev);
+        const uint64_t ends = m & ~((m >> 1) | (next << 63));
+        const int c = __popcll(starts);
         int inc = c;  // inclusive scan of start counts over the wave
         for (int o = 1; o < 64; o <<= 1) {
             int t = __shfl_up(inc, o, 64);
@@ -312,13 +340,13 @@ __global__ __launch_bounds__(256) void k_idx_write(const uint8_t* __restrict__ p
         for (int w = 0; w < wid; ++w) ex += wsum[w];
         const int total = wsum[0] + wsum[1] + wsum[2] + wsum[3];
         {
-            uint32_t s = starts;
+            uint64_t s = starts;
             int j = ex;
             while (s) {
-                const int b = __builtin_ctz(s);
+                const int b = __builtin_ctzll(s);
                 s &= s - 1;
-                st[j] = (uint16_t)(16 * tid + b);
-                if (out + j < cap) off[out + j] = base + 16 * tid + b;
+                st[j] = (uint16_t)(64 * tid + b);
+                if (out + j < cap) off[out + j] = base + 64 * tid + b;
                 ++j;
             }
         }
@@ -327,15 +355,16 @@ __global__ __launch_bounds__(256) void k_idx_write(const uint8_t* __restrict__ p
             // an end at local e closes the latest run started at or before e;
             // none started in the tile means the run began in an earlier tile,
             // whose block writes its length
-            uint32_t e = ends;
+            uint64_t e = ends;
             while (e) {
-                const int b = __builtin_ctz(e);
+                const int b = __builtin_ctzll(e);
                 e &= e - 1;
-                const int j = ex + __popc(starts & ((2u << b) - 1u)) - 1;
-                if (j >= 0 && out + j < cap) len[out + j] = 16 * tid + b + 1 - st[j];
+                // starts at or below bit b (2 << 63 wraps to 0: all bits)
+                const int j = ex + __popcll(starts & ((2ull << b) - 1ull)) - 1;
+                if (j >= 0 && out + j < cap) len[out + j] = 64 * tid + b + 1 - st[j];
             }
         }
-        if (tid == 255 && total > 0 && ((m >> 15) & 1u) && next) {
+        if (tid == 255 && total > 0 && (m >> 63) && next) {
             // the tile's last run continues past it: it ends at the first '@'
             // of a later tile, or at the end of the stream
             const int j = total - 1;
@@ -392,7 +421,7 @@ extern "C" int decompress_launcher(const uint8_t* d_in, uint8_t* d_out, int64_t 
     if ((reinterpret_cast<uintptr_t>(src) & 3) == 0 && (reinterpret_cast<uintptr_t>(d_out) & 15) == 0)
         chunks = n_syms / 32;
     if (chunks) {
-        hipLaunchKernelGGL(k_unpack32, dim3(grid_cap(chunks, 256)), dim3(256), 0, stream,
+        hipLaunchKernelGGL(k_unpack32, dim3(grid_cap(chunks, 256 * WIDE_U)), dim3(256), 0, stream,
                            reinterpret_cast<const uint32_t*>(src), reinterpret_cast<uint4*>(d_out),
                            chunks);
         if (int r = check_launch("decompress")) return r;
@@ -413,7 +442,7 @@ extern "C" int compress_launcher(const uint8_t* d_in, uint8_t* d_out, int64_t n_
     if ((reinterpret_cast<uintptr_t>(d_in) & 15) == 0 && (reinterpret_cast<uintptr_t>(d_out) & 3) == 0)
         chunks = n_syms / 32;
     if (chunks) {
-        hipLaunchKernelGGL(k_pack32, dim3(grid_cap(chunks, 256)), dim3(256), 0, stream,
+        hipLaunchKernelGGL(k_pack32, dim3(grid_cap(chunks, 256 * WIDE_U)), dim3(256), 0, stream,
                            reinterpret_cast<const uint4*>(d_in), reinterpret_cast<uint32_t*>(d_out),
                            chunks);
         if (int r = check_launch("compress")) return r;

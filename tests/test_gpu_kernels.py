@@ -132,11 +132,11 @@ def test_db_index_and_gather(orc, dev):
 
 
 def _index_streams(rng):
-    """Residue (1) / '@' (0) patterns around the index's 4096-symbol tiles:
-    runs spanning several tiles, runs that end or start on a tile edge, a
-    stream that is one run, alternation (2048 runs in a tile) and random
-    densities."""
-    T = 4096
+    """Residue (1) / '@' (0) patterns around the index's 16384-symbol tiles
+    (64 symbols per thread): runs spanning several tiles, runs that end or
+    start on a tile edge, a stream that is one run, alternation (8192 runs in
+    a tile) and random densities."""
+    T = 16384
     out = []
     one = np.ones(3 * T + 5, bool)
     out.append(one.copy())
@@ -150,8 +150,11 @@ def _index_streams(rng):
     out.append(~alt)
     for n, dens in ((T, 0.5), (T + 16, 0.02), (7 * T + 9, 0.003), (40 * T + 1, 0.0002), (1, 0.0)):
         out.append(rng.random(n) >= dens)
-    for n in (1, 15, 16, 17, T - 1, T, T + 1):
+    for n in (1, 15, 16, 17, 63, 64, 65, T - 1, T, T + 1):
         out.append(np.ones(n, bool))
+    s = one.copy()
+    s[63::64] = False  # every thread's last symbol
+    out.append(s)
     return out
 
 
