@@ -351,11 +351,38 @@ def main() -> int:
         log(json.dumps(res))
         return res
 
+    def measure_msa(query: str, calls: int = 2):
+        """The whole drop-in call, sseqs.msa(query, out.a3m) with the reference's
+        defaults (sseqs/msa.py:71-446) on the resident database: per-batch
+        cascade, final merge, device backtrack of the hits and the a3m file."""
+        import tempfile
+
+        from sseqs_amd.msa import msa
+        with tempfile.TemporaryDirectory() as d:
+            fn = os.path.join(d, "out.a3m")
+            msa(query, fn, db=db, verbose=False)
+            torch.cuda.synchronize()
+            t_search = t_a3m = 0.0
+            t_wall = time.perf_counter()
+            for _ in range(calls):
+                tm = msa(query, fn, db=db, verbose=False, return_timings=True, sync_time=True)
+                t_search += tm["t_search"]
+                t_a3m += tm["t_a3m"]
+            sec = (time.perf_counter() - t_wall) / calls
+            rows = sum(1 for line in open(fn) if line.startswith(">"))
+        res = {"q_len": len(query), "calls": calls, "ms_per_call": round(1e3 * sec, 1),
+               "search_ms": round(1e3 * t_search / calls, 1),
+               "a3m_ms": round(1e3 * t_a3m / calls, 1), "a3m_rows": rows,
+               "db_gb_per_s": round(db.nbytes / sec / 1e9, 2)}
+        log(json.dumps(res))
+        return res
+
     if args.batch_only:
         measure_batch(q200)
         return 0
     r200, launches200 = measure(q200)
     rbatch = measure_batch(q200) if args.batch_steps > 0 else None
+    re2e = measure_msa(q200) if args.batch_steps > 0 else None
     rates = kernel_rates(db, dev)
     log(f"stream kernels {json.dumps(rates)}")
     r500 = r500l = None
@@ -416,6 +443,7 @@ def main() -> int:
                          "kernel": dom["kernel"],
                          "algorithmic": "packed bytes 5/8*(L+1) per sequence of the launch"},
         "msa_batch_cascade": rbatch,
+        "msa_end_to_end": re2e,
         "stream_kernels": rates,
         "q500": r500,
         "q500_linear4": r500l,
