@@ -36,9 +36,11 @@ struct SrcAscii {
         int64_t prevp = gi > 0 ? (int64_t)P[gi - 1] : -1;  // P[-1] taken as -1
         int64_t st = prevp + 1, stop = (int64_t)P[gi] + 1;
         int64_t length = stop - st - 1;
-        start = st;
-        len = length > 0 ? (int)length : 0;
-        return !(length == 0 || stop < st);
+        const bool ok = !(length == 0 || stop < st);
+        // a skipped target reports its delimiter, which lies inside the batch
+        start = ok ? st : (int64_t)P[gi];
+        len = ok && length > 0 ? (int)length : 0;
+        return ok;
     }
     __device__ uint32_t sym(int64_t start, int p) const { return ascii[start + p]; }
 };
@@ -199,12 +201,14 @@ __global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
     __shared__ int8_t s_bl[400];
     __shared__ uint4 s_prof[21 * NW * 16];
     __shared__ uint8_t s_res[SW16_WAVES][4][SW_CH];
+    __shared__ uint8_t s_code[256];  // byte -> residue index (aa_index)
 
     const int tid = threadIdx.x;
     const int wid = tid >> 6, lane = tid & 63;
     const int grp = lane >> 4, l = lane & 15;
 
     for (int i = tid; i < 400; i += 64 * SW16_WAVES) s_bl[i] = bl.v[i];
+    for (int i = tid; i < 256; i += 64 * SW16_WAVES) s_code[i] = (uint8_t)aa_index((uint32_t)i);
     __syncthreads();
     for (int idx = tid; idx < 21 * NW * 16; idx += 64 * SW16_WAVES) {
         const int c = idx / (NW * 16);
@@ -261,9 +265,20 @@ __global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
             // the rows are overwritten, and the writes before the reads
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
-            for (int r = l; r < rows; r += 16) {
-                const int p = c0 + r;
-                res[r] = (uint8_t)(p < T ? aa_index(src.sym(start, p)) : 20);
+            // four rows per lane per step, loads first; a row past the
+            // target's end re-reads its last residue (or, for an empty or
+            // skipped target, the symbol at `start`, which is in bounds) and
+            // stores 20
+            for (int r0 = l; r0 < rows; r0 += 64) {
+                uint32_t sy[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    sy[u] = src.sym(start, max(min(c0 + r0 + 16 * u, T - 1), 0));
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int r = r0 + 16 * u;
+                    if (r < rows) res[r] = c0 + r < T ? s_code[sy[u] & 255u] : (uint8_t)20;
+                }
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
