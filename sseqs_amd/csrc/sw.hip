@@ -186,6 +186,12 @@ constexpr int SW16_WAVES = 4;
 __device__ inline i16x2 as_i16x2(uint32_t u) { return __builtin_bit_cast(i16x2, u); }
 __device__ inline uint32_t as_u32(i16x2 v) { return __builtin_bit_cast(uint32_t, v); }
 __device__ inline i16x2 vmax(i16x2 a, i16x2 b) { return __builtin_elementwise_max(a, b); }
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+// max(a - b, 0) for a, b >= 0 in one instruction (unsigned saturating subtract)
+__device__ inline i16x2 subsat(i16x2 a, u16x2 b) {
+    return __builtin_bit_cast(
+        i16x2, __builtin_elementwise_sub_sat(__builtin_bit_cast(u16x2, a), b));
+}
 
 template <int CTRL>
 __device__ inline int sw_dpp(int v) {
@@ -238,6 +244,9 @@ __global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
     const i16x2 zero = {0, 0};
     // affine: open, extend; linear: p1 = gap (<= 0)
     const i16x2 c1 = {(short)p1, (short)p1}, c2 = {(short)p2, (short)p2};
+    // the gap costs as unsigned amounts: extend (affine), -gap (linear)
+    const u16x2 u2 = {(unsigned short)p2, (unsigned short)p2};
+    const u16x2 ug = {(unsigned short)-p1, (unsigned short)-p1};
     uint8_t* res = s_res[wid][grp];
     for (int64_t quad = (int64_t)blockIdx.x * SW16_WAVES + wid; quad * 4 < n;
          quad += (int64_t)gridDim.x * SW16_WAVES) {
@@ -304,7 +313,9 @@ __global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
                                 const i16x2 sc = as_i16x2(pv[b]);
                                 const i16x2 m = dg + sc;
                                 const i16x2 x = vmax(lM - c1, lX - c2);
-                                const i16x2 y = vmax(vmax(Mp[k] - c1, Yp[k] - c2), zero);
+                                // max(Mp - open, Yp - extend, 0): Yp >= 0, so
+                                // the saturated second term carries the 0
+                                const i16x2 y = vmax(Mp[k] - c1, subsat(Yp[k], u2));
                                 const i16x2 h = vmax(m, vmax(x, y));
                                 dg = Hp[k];
                                 Hp[k] = h;
@@ -327,9 +338,10 @@ __global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
                             const int k = 4 * w + b;
                             if (k < TPT) {
                                 const i16x2 sc = as_i16x2(pv[b]);
-                                // the chain through `left` is three instructions
-                                const i16x2 h =
-                                    vmax(vmax(dg + sc, zero), vmax(Hp[k], left) + c1);
+                                // max(dg + sc, Hp + gap, left + gap, 0): Hp and
+                                // left are >= 0, so max(Hp, left) + gap
+                                // saturated at 0 carries the 0
+                                const i16x2 h = vmax(dg + sc, subsat(vmax(Hp[k], left), ug));
                                 dg = Hp[k];
                                 Hp[k] = h;
                                 left = h;
