@@ -258,6 +258,7 @@ __global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
         const int T = valid ? len : 0;
         int Tmax = max(T, __shfl_xor(T, 16, 64));
         Tmax = max(Tmax, __shfl_xor(Tmax, 32, 64));
+        Tmax = __builtin_amdgcn_readfirstlane(Tmax);  // uniform: loops run on scalars
 
         i16x2 Hp[TPT], Mp[TPT], Yp[TPT];
 #pragma unroll
@@ -291,7 +292,9 @@ __global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
-            for (int r = 0; r < rows; ++r) {
+            // one row of the DP; two per step so that the register
+            // rotation of Hp between rows needs no copies
+            auto row = [&](const int r) {
                 const int c = res[r];
                 const uint4* pr = &s_prof[c * NW * 16 + l];
                 const uint32_t rot = (uint32_t)sw_dpp<0x121>((int)as_u32(Hp[TPT - 1]));
@@ -350,7 +353,13 @@ __global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
                         }
                     }
                 }
+            };
+            int r = 0;
+            for (; r + 1 < rows; r += 2) {
+                row(r);
+                row(r + 1);
             }
+            if (r < rows) row(r);
         }
         int bv = max((int)best.x, (int)best.y);
         bv = max(bv, sw_dpp<0x128>(bv));
