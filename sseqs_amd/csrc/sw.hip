@@ -242,6 +242,9 @@ __global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
     __syncthreads();
 
     const i16x2 zero = {0, 0};
+    // byte selector of the diagonal operand: the rotated word as it is, or
+    // for lane 0 (stripes 0 and 16) 0 below and the word's low half above
+    const uint32_t dg_sel = l == 0 ? 0x01000c0cu : 0x03020100u;
     // affine: open, extend; linear: p1 = gap (<= 0)
     const i16x2 c1 = {(short)p1, (short)p1}, c2 = {(short)p2, (short)p2};
     // the gap costs as unsigned amounts: extend (affine), -gap (linear)
@@ -297,8 +300,11 @@ __global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
             auto row = [&](const int r) {
                 const int c = res[r];
                 const uint4* pr = &s_prof[c * NW * 16 + l];
-                const uint32_t rot = (uint32_t)sw_dpp<0x121>((int)as_u32(Hp[TPT - 1]));
-                i16x2 dg = as_i16x2(l == 0 ? rot << 16 : rot);
+                // row rotation by one lane (every lane of the row reads, so
+                // no old value is needed), then lane 0's halves rearranged
+                const uint32_t rot = (uint32_t)__builtin_amdgcn_mov_dpp(
+                    (int)as_u32(Hp[TPT - 1]), 0x121, 0xf, 0xf, false);
+                i16x2 dg = as_i16x2(__builtin_amdgcn_perm(rot, rot, dg_sel));
                 if constexpr (AFFINE) {
                     i16x2 lM = zero, lX = zero;
 #pragma unroll
