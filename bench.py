@@ -193,6 +193,9 @@ def main() -> int:
                     help="run only the q_len=200 prefilter sweep --steps times (for counters)")
     ap.add_argument("--stream-only", action="store_true",
                     help="time only the unpack, pack and index kernels (for traces)")
+    ap.add_argument("--shuffled", action="store_true",
+                    help="time the q_len=200 prefilter on the database and on a copy of its stream in "
+                         "random order (use with --n-seqs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="run a plain device copy of BYTES bytes and exit (counter calibration)")
     args = ap.parse_args()
@@ -222,6 +225,26 @@ def main() -> int:
     torch.cuda.synchronize()
     log(f"generated {db.n} sequences, {db.residues} residues, {db.nbytes / 1e9:.2f} GB packed "
         f"in {time.time() - t0:.1f} s")
+    if args.shuffled:
+        sh, perm = synth.shuffled_copy(db)
+        q = prepare(q200, device=dev)
+        res = {}
+        for name, d in (("length_sorted_stream", db), ("shuffled_stream", sh)):
+            s = prefilter_scores(d, q)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(args.steps):
+                s = prefilter_scores(d, q)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t) / args.steps * 1e3
+            res[name] = {"ms": round(ms, 3), "db_gb_per_s": round(d.nbytes / ms / 1e6, 1)}
+            res[name + "_scores"] = s
+        back = perm[sh.orig_id] if sh.orig_id is not None else perm
+        same = torch.equal(res.pop("length_sorted_stream_scores")[back],
+                           res.pop("shuffled_stream_scores"))
+        print(json.dumps({"prefilter_q200_shuffled": res, "n_seqs": db.n,
+                          "scores_identical": bool(same)}), flush=True)
+        return 0
     if args.stream_only:
         print(json.dumps({"stream_kernels": kernel_rates(db, dev)}), flush=True)
         return 0

@@ -193,3 +193,27 @@ def generate_db(n: int, queries: list[str], seed: int = 0, device="cuda",
          ptr(d_sym), len(cls.length), n, n_syms, ptr(packed), ptr(seq_off), ptr(seq_len),
          ptr(homq), ptr(ws), ws.numel(), stream_of(dev))
     return SyntheticDB(packed, seq_off, seq_len, homq, cls, n_syms)
+
+
+def shuffled_copy(db, seed: int = 3):
+    """The sequences of ``db`` written to a new packed stream in random order
+    and indexed by ``SeqDB.from_packed``, which sorts the index by length:
+    the layout of a database file that is not in length order.  Returns
+    (database, perm): stream sequence i of the copy is sequence perm[i] of
+    ``db``, so index entry j of the copy is sequence perm[orig_id[j]]."""
+    from . import ops
+    from .db import SeqDB
+    dev = db.device
+    asc = ops.decompress(db.packed, db.n_syms, 0)
+    g = torch.Generator(device=dev)
+    g.manual_seed(seed)
+    perm = torch.randperm(db.n, device=dev, generator=g)
+    ln = db.seq_len[perm].to(torch.int64)
+    out_off = torch.cumsum(ln + 1, 0) - (ln + 1)
+    total = int((ln + 1).sum().item())
+    seg = torch.repeat_interleave(torch.arange(db.n, device=dev), ln)
+    pos = torch.arange(seg.numel(), device=dev) - (torch.cumsum(ln, 0) - ln)[seg]
+    out = torch.full((total,), 64, dtype=torch.uint8, device=dev)
+    out[out_off[seg] + pos] = asc[db.seq_off[perm][seg] + pos]
+    del seg, pos, asc
+    return SeqDB.from_packed(ops.compress(out), total), perm

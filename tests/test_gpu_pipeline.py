@@ -75,6 +75,34 @@ def test_global_cascade(orc, world, qi):
     assert np.mean(hq[cand[top[:50]]] == qi) > 0.9
 
 
+def test_unsorted_stream(world):
+    """A database file whose sequences are not in length order: the copy's
+    index (sorted by length, with orig_id back to the stream) names the same
+    sequences, the prefilter gives bit-identical scores per sequence (its
+    tiles now read scattered stream positions), and the global cascade
+    selects the same candidates and hit scores."""
+    from sseqs_amd import synth
+    from sseqs_amd.query import prepare
+    from sseqs_amd.search import global_search, prefilter_scores
+    announce("unsorted stream")
+    db, query = world["db"], world["queries"][0]
+    sh, perm = synth.shuffled_copy(db)
+    assert sh.n == db.n and sh.orig_id is not None
+    back = perm[sh.orig_id]  # index entry j of the copy is sequence back[j] of db
+    assert torch.equal(sh.seq_len, db.seq_len[back])
+    ids = torch.arange(0, sh.n, 97, device=sh.device)
+    assert sh.strings(ids, 1200) == db.strings(back[ids], 1200)
+    q = prepare(query)
+    s_db, s_sh = prefilter_scores(db, q), prefilter_scores(sh, q)
+    assert torch.equal(s_db[back], s_sh)
+    top = torch.sort(s_db, descending=True).values
+    assert float(top[2999]) != float(top[3000])  # no tie at the selection threshold
+    g1 = global_search(db, q, top_prefilter=3000, top_hits=500)
+    g2 = global_search(sh, q, top_prefilter=3000, top_hits=500)
+    assert torch.equal(torch.sort(back[g2.candidates]).values, torch.sort(g1.candidates).values)
+    assert torch.equal(torch.sort(g2.scores).values, torch.sort(g1.scores).values)
+
+
 def test_global_cascade_linear_gap(orc, world):
     """The linear-gap (-4) run of the whole-database cascade: the same
     candidates, scored by linear SW bit-exactly."""
