@@ -242,18 +242,6 @@ def main() -> int:
         back = perm[sh.orig_id] if sh.orig_id is not None else perm
         same = torch.equal(res.pop("length_sorted_stream_scores")[back],
                            res.pop("shuffled_stream_scores"))
-        # msa()'s per-batch cascade: batches follow the stream, so on the
-        # shuffled copy every batch mixes all lengths
-        from sseqs_amd.search import batch_search
-        for name, d in (("length_sorted_stream", db), ("shuffled_stream", sh)):
-            batch_search(d, q200, q, bs=5_000_000)
-            torch.cuda.synchronize()
-            t = time.perf_counter()
-            for _ in range(args.steps):
-                batch_search(d, q200, q, bs=5_000_000)
-            torch.cuda.synchronize()
-            ms = (time.perf_counter() - t) / args.steps * 1e3
-            res[name]["batch_cascade_ms"] = round(ms, 3)
         print(json.dumps({"prefilter_q200_shuffled": res, "n_seqs": db.n,
                           "scores_identical": bool(same)}), flush=True)
         return 0
