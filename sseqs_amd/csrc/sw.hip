@@ -263,11 +263,13 @@ __global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
         Tmax = max(Tmax, __shfl_xor(Tmax, 32, 64));
         Tmax = __builtin_amdgcn_readfirstlane(Tmax);  // uniform: loops run on scalars
 
-        i16x2 Hp[TPT], Mp[TPT], Yp[TPT];
+        // Mo holds M - open of the previous row: the one value both the next
+        // row (Y) and the next column (X) take from M
+        i16x2 Hp[TPT], Mo[TPT], Yp[TPT];
 #pragma unroll
         for (int k = 0; k < TPT; ++k) {
             Hp[k] = zero;
-            Mp[k] = zero;
+            Mo[k] = zero - c1;
             Yp[k] = zero;
         }
         i16x2 best = zero;
@@ -306,7 +308,7 @@ __global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
                     (int)as_u32(Hp[TPT - 1]), 0x121, 0xf, 0xf, false);
                 i16x2 dg = as_i16x2(__builtin_amdgcn_perm(rot, rot, dg_sel));
                 if constexpr (AFFINE) {
-                    i16x2 lM = zero, lX = zero;
+                    i16x2 lMo = zero - c1, lX = zero;
 #pragma unroll
                     for (int w = 0; w < NW; ++w) {
                         const uint4 pw = pr[w * 16];
@@ -321,16 +323,17 @@ __global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
                                 // the clamped recurrence's H
                                 const i16x2 sc = as_i16x2(pv[b]);
                                 const i16x2 m = dg + sc;
-                                const i16x2 x = vmax(lM - c1, lX - c2);
-                                // max(Mp - open, Yp - extend, 0): Yp >= 0, so
+                                const i16x2 x = vmax(lMo, lX - c2);
+                                // max(M_up - open, Yp - extend, 0): Yp >= 0, so
                                 // the saturated second term carries the 0
-                                const i16x2 y = vmax(Mp[k] - c1, subsat(Yp[k], u2));
+                                const i16x2 y = vmax(Mo[k], subsat(Yp[k], u2));
                                 const i16x2 h = vmax(m, vmax(x, y));
+                                const i16x2 mo = m - c1;
                                 dg = Hp[k];
                                 Hp[k] = h;
-                                Mp[k] = m;
+                                Mo[k] = mo;
                                 Yp[k] = y;
-                                lM = m;
+                                lMo = mo;
                                 lX = x;
                                 best = vmax(best, h);
                             }
