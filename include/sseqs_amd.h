@@ -104,7 +104,8 @@ int launch_sw_cuda_affine(const uint8_t* query_seq_indices_ptr, int query_length
  * seq_off (symbol offset of the first residue) and seq_len, and the number
  * of sequences found to *d_nseq (device int64).  Empty runs (consecutive
  * '@') are skipped, matching the `lengths != 0` filter of sseqs/msa.py:253.
- * ws must hold sseqs_db_index_ws_bytes(n_syms) bytes.
+ * ws must hold sseqs_db_index_ws_bytes(n_syms) bytes; packed must be 4-byte
+ * aligned.  With cap == 0 only the count is written.
  * Database-resident form of the delimiter scan of sseqs/msa.py:227-267. */
 size_t sseqs_db_index_ws_bytes(int64_t n_syms);
 int sseqs_db_index(const uint8_t* packed, int64_t n_syms, int64_t* seq_off, int32_t* seq_len,
