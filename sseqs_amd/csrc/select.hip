@@ -66,26 +66,57 @@ __global__ __launch_bounds__(256) void k_sel_hist(const void* s, int64_t n,
         if (lh[i]) atomicAdd(&hist[i], lh[i]);
 }
 
-// One block: walk bins from the top, fix the next digit of T, clear hist.
+// One block: fix the next digit of T, clear hist.  The digit is the bin b
+// (from the top) where the keys above it fall short of k_rem and the keys
+// above plus those in it do not; the counts above every bin come from one
+// block-wide scan of the bins in descending order.
 __global__ __launch_bounds__(1024) void k_sel_pick(SelState* st, uint32_t* hist, int shift,
                                                    int bits) {
-    __shared__ uint32_t cnt[SEL_BINS];
+    __shared__ uint32_t wsum[16];
+    __shared__ int pick_b;
+    __shared__ uint32_t pick_above, cnt0;
     const int nb = 1 << bits;
-    for (int i = threadIdx.x; i < SEL_BINS; i += 1024) {
-        cnt[i] = i < nb ? hist[i] : 0u;
-        hist[i] = 0u;
+    const int t = threadIdx.x, lane = t & 63, wid = t >> 6;
+    // thread t holds bins hi = nb-1-2t and lo = hi-1, so ascending t walks
+    // the bins from the top
+    const int hi = nb - 1 - 2 * t, lo = hi - 1;
+    const uint32_t c_hi = hi >= 0 ? hist[hi] : 0u, c_lo = lo >= 0 ? hist[lo] : 0u;
+    if (t == 0) pick_b = -1;
+    __syncthreads();
+    if (hi >= 0) hist[hi] = 0u;
+    if (lo >= 0) hist[lo] = 0u;
+    if (lo == 0) cnt0 = c_lo;
+    const uint32_t c = c_hi + c_lo;
+    uint32_t inc = c;  // inclusive scan over the wave
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t v = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += v;
+    }
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    uint32_t above = inc - c;  // keys in the bins above hi
+    for (int w = 0; w < wid; ++w) above += wsum[w];
+    const uint32_t need = st->k_rem;
+    if (hi >= 0 && above < need && need <= above + c_hi) {
+        pick_b = hi;
+        pick_above = above;
+    } else if (lo >= 0 && above + c_hi < need && need <= above + c) {
+        pick_b = lo;
+        pick_above = above + c_hi;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t need = st->k_rem, above = 0;
-        int b = nb - 1;
-        for (; b > 0; --b) {
-            if (above + cnt[b] >= need) break;
-            above += cnt[b];
+    if (t == 0) {
+        int b = pick_b;
+        uint32_t ab = pick_above;
+        if (b < 0) {  // fewer keys than k_rem: bin 0, with every other bin above it
+            b = 0;
+            ab = 0;
+            for (int w = 0; w < 16; ++w) ab += wsum[w];
+            ab -= cnt0;
         }
         st->prefix |= (uint32_t)b << shift;
         st->mask |= ((1u << bits) - 1u) << shift;
-        st->k_rem = need - above;
+        st->k_rem = need - ab;
     }
 }
 
