@@ -185,6 +185,8 @@ def main() -> int:
     ap.add_argument("--top-hits", type=int, default=4096)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-q500", action="store_true")
+    ap.add_argument("--no-recall", action="store_true",
+                    help="skip Recall@4096 against exhaustive SW of the whole database")
     ap.add_argument("--batch-steps", type=int, default=2,
                     help="timed passes of the per-batch cascade (0 skips it)")
     ap.add_argument("--batch-only", action="store_true",
@@ -283,6 +285,57 @@ def main() -> int:
                            linear_gap=linear_gap, events=events)
         al = backtrack_hits(db, q, gh.ids, bt_ws)
         return gh, al
+
+    def recall(query: str, k: int = 4096, chunk: int = 25_000_000):
+        """Recall@k of the plain cascade (prefilter top --top-prefilter, affine
+        SW, top k) against affine SW of every database sequence: the share of
+        the cascade's k hits whose score reaches the k-th exhaustive score
+        (hits tied with it count as found)."""
+        from sseqs_amd.search import sw_affine_ids
+        q = prepare(query, device=dev)
+        gh = global_search(db, q, args.top_prefilter, k, scores_buf=scores_buf)
+        allsc = torch.empty(db.n, dtype=torch.int32, device=dev)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for b in range(0, db.n, chunk):
+            ids = torch.arange(b, min(db.n, b + chunk), device=dev)
+            allsc[b: b + ids.numel()] = sw_affine_ids(db, q, ids)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t
+        kth = int(torch.topk(allsc, k).values[-1].item())
+        found = min(k, int((gh.scores >= kth).sum().item()))
+        cells = len(query) * db.residues
+        # recall at every candidate count K: the cascade keeps the K best
+        # prefilter scores (index order among ties), and finds a hit when one
+        # of them reaches the k-th exhaustive score
+        order = torch.sort(gh.prefilter, descending=True, stable=True).indices
+        cum = torch.cumsum((allsc >= kth)[order].to(torch.int32), 0)
+        del order, allsc
+        curve = {}
+        for K in (100_000, 300_000, 1_000_000, 3_000_000, 10_000_000, 30_000_000, 100_000_000):
+            if K <= db.n:
+                curve[str(K)] = round(min(k, int(cum[K - 1].item())) / k, 4)
+        need = -(-99 * k // 100)
+        k99 = int(torch.searchsorted(cum, torch.tensor([need], dtype=cum.dtype, device=dev)).item()) + 1
+        del cum
+        # the plain cascade run at that operating point
+        global_search(db, q, k99, k, scores_buf=scores_buf)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        g99 = global_search(db, q, k99, k, scores_buf=scores_buf)
+        torch.cuda.synchronize()
+        ms99 = 1e3 * (time.perf_counter() - t)
+        r99 = min(k, int((g99.scores >= kth).sum().item())) / k
+        res = {"q_len": len(query), "k": k, "top_prefilter": args.top_prefilter,
+               "recall": round(found / k, 4), "kth_exhaustive_score": kth,
+               "recall_by_top_prefilter": curve,
+               "top_prefilter_for_0.99": k99, "share_of_db_for_0.99": round(k99 / db.n, 4),
+               "cascade_ms_at_0.99": round(ms99, 1), "recall_measured_at_0.99": round(r99, 4),
+               "search_gcups_at_0.99": round(cells / ms99 / 1e6, 1),
+               "exhaustive_sw_ms": round(1e3 * sec, 1),
+               "exhaustive_sw_gcups": round(cells / sec / 1e9, 1)}
+        log(json.dumps(res))
+        return res
 
     def measure(query: str, linear_gap=None):
         q = prepare(query, device=dev)
@@ -408,10 +461,13 @@ def main() -> int:
     re2e = measure_msa(q200) if args.batch_steps > 0 else None
     rates = kernel_rates(db, dev)
     log(f"stream kernels {json.dumps(rates)}")
+    recalls = [recall(q200)] if not args.no_recall else []
     r500 = r500l = None
     if not args.no_q500:
         r500, _ = measure(q500)
         r500l, _ = measure(q500, linear_gap=-4)
+        if not args.no_recall:
+            recalls.append(recall(q500))
     cpu = None
     if not args.no_cpu:
         t0 = time.time()
@@ -467,6 +523,7 @@ def main() -> int:
                          "algorithmic": "packed bytes 5/8*(L+1) per sequence of the launch"},
         "msa_batch_cascade": rbatch,
         "msa_end_to_end": re2e,
+        "recall_at_4096": recalls,
         "stream_kernels": rates,
         "q500": r500,
         "q500_linear4": r500l,
