@@ -297,7 +297,7 @@ __global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
             __builtin_amdgcn_wave_barrier();
-            // one row of the DP; two per step so that the register
+            // one row of the DP; several per step so that the register
             // rotation of Hp between rows needs no copies
             auto row = [&](const int r) {
                 const int c = res[r];
@@ -363,12 +363,15 @@ __global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
                     }
                 }
             };
+            // short stripes: four rows per step (the per-row overhead is a
+            // larger share of a short row); long stripes keep their registers
+            constexpr int RS = TPT <= 8 ? 4 : 2;
             int r = 0;
-            for (; r + 1 < rows; r += 2) {
-                row(r);
-                row(r + 1);
+            for (; r + RS - 1 < rows; r += RS) {
+#pragma unroll
+                for (int u = 0; u < RS; ++u) row(r + u);
             }
-            if (r < rows) row(r);
+            for (; r < rows; ++r) row(r);
         }
         int bv = max((int)best.x, (int)best.y);
         bv = max(bv, sw_dpp<0x128>(bv));
