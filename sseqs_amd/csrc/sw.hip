@@ -69,11 +69,13 @@ __global__ __launch_bounds__(64) void k_sw(Src src, const uint8_t* __restrict__ 
     __shared__ int8_t s_bl[400];
     __shared__ uint32_t s_prof[21 * 32 * NW];
     __shared__ uint8_t s_res[2][SW_CH];
+    __shared__ uint8_t s_code[256];  // byte -> residue index (aa_index)
 
     const int tid = threadIdx.x;
     const int half = tid >> 5, lane = tid & 31;
 
     for (int i = tid; i < 400; i += 64) s_bl[i] = bl.v[i];
+    for (int i = tid; i < 256; i += 64) s_code[i] = (uint8_t)aa_index((uint32_t)i);
     __syncthreads();
     // profile word w of (residue c, lane ln) packs columns ln*TPT + 4w .. +3
     for (int idx = tid; idx < 21 * 32 * NW; idx += 64) {
@@ -114,9 +116,20 @@ __global__ __launch_bounds__(64) void k_sw(Src src, const uint8_t* __restrict__ 
         for (int c0 = 0; c0 < Tmax; c0 += SW_CH) {
             const int rows = min(SW_CH, Tmax - c0);
             __syncthreads();
-            for (int r = lane; r < rows; r += 32) {
-                int p = c0 + r;
-                s_res[half][r] = (uint8_t)(p < T ? aa_index(src.sym(start, p)) : 20);
+            // four rows per lane per step, loads first; a row past the
+            // target's end re-reads its last residue (or, for an empty or
+            // skipped target, the symbol at `start`, which is in bounds) and
+            // stores 20
+            for (int r0 = lane; r0 < rows; r0 += 128) {
+                uint32_t sy[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    sy[u] = src.sym(start, max(min(c0 + r0 + 32 * u, T - 1), 0));
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int r = r0 + 32 * u;
+                    if (r < rows) s_res[half][r] = c0 + r < T ? s_code[sy[u] & 255u] : (uint8_t)20;
+                }
             }
             __syncthreads();
             for (int r = 0; r < rows; ++r) {
