@@ -177,6 +177,18 @@ def test_batch_cascade_configs(orc, world, bs, qi, top_fft, top_sw):
                          top_fft=top_fft, top_sw=top_sw, top_sw_affine=2)
 
 
+def test_batch_cascade_unsorted_stream(orc, world):
+    """The per-batch cascade on a database file not in length order: every
+    batch mixes all lengths, so its prefilter tiles hold sequences of very
+    different lengths and every batch takes the largest N."""
+    from sseqs_amd import synth
+    announce("batch cascade unsorted stream")
+    sh, _ = synth.shuffled_copy(world["db"])
+    ascii = orc.xbit_unpack(sh.packed.cpu().numpy(), sh.n_syms)
+    _check_batch_cascade(orc, sh, ascii, world["queries"][0], 1_000_000, True,
+                         top_fft=50, top_sw=5, top_sw_affine=2)
+
+
 def test_batch_cascade_shifted_segments(orc, dev):
     """Batches that start on '@' or hold '@@': the reference's `lengths != 0`
     filter shifts the offsets of later segments; both paths reproduce it."""
