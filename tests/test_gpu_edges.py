@@ -1,0 +1,86 @@
+"""Degenerate inputs through the operator mirror and the Python API, on the
+GPU: empty inputs, selections of none and of every score, targets and
+queries without residues, compared with the oracle where it defines them."""
+import numpy as np
+import pytest
+import torch
+
+from conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+
+def test_codec_empty(dev):
+    from sseqs_amd import ops
+    announce("codec empty")
+    assert ops.compress(torch.empty(0, dtype=torch.uint8, device=dev)).numel() == 0
+    assert ops.decompress(torch.zeros(5, dtype=torch.uint8, device=dev), 0, 0).numel() == 0
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float16, torch.int32])
+def test_topk_none_and_all(dev, dtype):
+    """k = 0 gives nothing; k = n gives every index, in (score desc, index
+    asc) order when sorted and in index order otherwise, with many ties."""
+    from sseqs_amd import ops
+    announce(f"topk none/all {dtype}")
+    x = np.random.default_rng(4).integers(-50, 50, 3001)
+    s = torch.tensor(x, dtype=dtype, device=dev)
+    assert ops.topk_indices(s, 0).numel() == 0
+    got = ops.topk_indices(s, x.size, sorted=True).cpu().numpy()
+    assert np.array_equal(got, np.lexsort((np.arange(x.size), -x)))
+    got = ops.topk_indices(s, x.size, sorted=False).cpu().numpy()
+    assert np.array_equal(got, np.arange(x.size))
+
+
+def test_sw_operators_degenerate(orc, dev):
+    """No targets; a query without residues (every target scores -1,
+    sseqs/sw.cu:73-77); targets of non-residues only, an empty target
+    between two delimiters and a target at index 0."""
+    from sseqs_amd import ops
+    announce("sw operators degenerate")
+    a = np.frombuffer(b"ACDEWY@XXXX@@BZJ@MKVLLA@", dtype=np.uint8).copy()
+    D = np.nonzero(a == 64)[0].astype(np.int32)
+    qc = orc.encode_query_sw("MKVLLAACDE")
+    at, Dt = torch.from_numpy(a).to(dev), torch.from_numpy(D).to(dev)
+    qt = torch.from_numpy(qc).to(dev)
+    none = torch.empty(0, dtype=torch.int32, device=dev)
+    assert ops.sw_cuda_blosum62(qt, none, at, Dt).numel() == 0
+    assert ops.sw_cuda_affine(qt, none, at, Dt).numel() == 0
+    gi = np.array([0, 1, 2, 3, 4], dtype=np.int32)
+    git = torch.from_numpy(gi).to(dev)
+    for gap in (-1, -4):
+        got = ops.sw_cuda_blosum62(qt, git, at, Dt, gap_penalty_param=gap).cpu().numpy()
+        assert np.array_equal(got, orc.sw_linear(qc, gi, a, D, gap)), gap
+    starts = (D + 1)[D + 1 < a.size].astype(np.int32)
+    got = ops.sw_cuda_affine(qt, git[:4], at, torch.from_numpy(starts).to(dev)).cpu().numpy()
+    assert np.array_equal(got, orc.sw_affine(qc, gi[:4], a, starts, 11, 1))
+    q0 = torch.empty(0, dtype=torch.uint8, device=dev)
+    assert ops.sw_cuda_blosum62(q0, git, at, Dt).tolist() == [-1] * gi.size
+    assert ops.sw_cuda_affine(q0, git, at, Dt).tolist() == [-1] * gi.size
+
+
+def test_batch_cascade_query_without_residues(orc, dev):
+    """A query of letters outside the 20 residues: the prefilter scores it
+    through the unknown-residue code and both SW stages score -1, so the
+    hits are whatever the selections keep, as in the reference."""
+    from sseqs_amd import synth
+    from sseqs_amd.db import SeqDB
+    from sseqs_amd.query import prepare
+    from sseqs_amd.search import batch_search
+    announce("batch cascade query without residues")
+    db = SeqDB.synthetic(4000, [synth.make_query(80, 0, seed=7)], seed=8, device=dev)
+    ascii = orc.xbit_unpack(db.packed.cpu().numpy(), db.n_syms)
+    query = "XBZJ" * 12
+    dev_scores = {}
+    got = batch_search(db, query, prepare(query), bs=150_000, top_fft=20, top_sw=4,
+                       on_scores=lambda j, s: dev_scores.__setitem__(j, s.float().cpu().numpy()))
+    want, fft64 = orc.batch_cascade(ascii, query, 150_000 * 8 // 5, top_fft=20, top_sw=4,
+                                    fft_scores=lambda j: dev_scores[j])
+    for j, w in enumerate(fft64):
+        if w.size:
+            scale = 1e-3 * max(1.0, float(np.max(np.abs(w))))
+            assert np.all(np.abs(dev_scores[j] - w) <= 1e-2 * np.abs(w) + scale), j
+    assert len(got) == len(want) > 0
+    for g, w in zip(got, want):
+        assert (np.float32(g.norm), g.affine, g.linear, g.seq) == w
+        assert g.affine == -1 and g.linear == -1
