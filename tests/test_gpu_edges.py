@@ -59,7 +59,7 @@ def test_sw_operators_degenerate(orc, dev):
     assert ops.sw_cuda_affine(q0, git, at, Dt).tolist() == [-1] * gi.size
 
 
-def test_batch_cascade_query_without_residues(orc, dev):
+def test_batch_cascade_query_without_residues(orc, dev, tmp_path):
     """A query of letters outside the 20 residues: the prefilter scores it
     through the unknown-residue code and both SW stages score -1, so the
     hits are whatever the selections keep, as in the reference."""
@@ -84,3 +84,11 @@ def test_batch_cascade_query_without_residues(orc, dev):
     for g, w in zip(got, want):
         assert (np.float32(g.norm), g.affine, g.linear, g.seq) == w
         assert g.affine == -1 and g.linear == -1
+    # and the whole call: merge, backtrack (non-residues score -5) and a3m
+    import sseqs_amd
+    from sseqs_amd.search import rank_hits
+    out = tmp_path / "x.a3m"
+    sseqs_amd.msa(query, str(out), num_msas=64, verbose=False, db=db, bs=150_000, top_fft=20,
+                  top_sw=4)
+    hits = rank_hits(got, 64)
+    assert out.read_text() == orc.a3m(query, [orc.backtrack(query, h.seq) for h in hits])
