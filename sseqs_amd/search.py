@@ -19,7 +19,6 @@ unspecified in the reference).
 from __future__ import annotations
 
 import ctypes
-import math
 import time
 from dataclasses import dataclass, field
 

@@ -10,7 +10,6 @@
 * Invariants at realistic sizes: stripe score <= textbook score, equality
   when horizontal gaps can not pay off, query-symmetric inputs.
 """
-import itertools
 
 import numpy as np
 import pytest
