@@ -124,6 +124,18 @@ int sseqs_db_prefilter(const uint8_t* packed, const int64_t* seq_off, const int3
                        const float* lutA, const float* lutB, int vocab_size, int k, int N,
                        int maxlen, float* scores, hipStream_t stream);
 
+/* sseqs_db_prefilter for (offset, length) spans that may hold '@' inside:
+ * the per-batch cascade's segments of a batch whose `lengths != 0` filter
+ * shifted its offsets (sseqs/msa.py:253,266-267), where the reference's
+ * ASCII prefilter scores an in-span '@' through LUT code 21
+ * (sseqs/fft.cu:5-15,91-93).  Always uses the generic kernels, which do the
+ * same; arguments as sseqs_db_prefilter. */
+int sseqs_db_prefilter_spans(const uint8_t* packed, const int64_t* seq_off,
+                             const int32_t* seq_len, int64_t s_begin, int64_t s_end,
+                             const uint8_t* qrev, int query_len, const float* lutA,
+                             const float* lutB, int vocab_size, int k, int N, int maxlen,
+                             float* scores, hipStream_t stream);
+
 /* Name of the kernel sseqs_db_prefilter launches for these sizes (for
  * profiles and reports), written to name[0..name_len).  Host only. */
 int sseqs_db_prefilter_kernel(int query_len, int k, int N, int maxlen, char* name, int name_len);

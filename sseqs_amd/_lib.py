@@ -48,6 +48,8 @@ SIGNATURES = {
     "sseqs_db_index": (_i32, [_vp, _i64, _vp, _vp, _i64, _vp, _vp, _sz, _vp]),
     "sseqs_db_prefilter": (_i32, [_vp, _vp, _vp, _i64, _i64, _vp, _i32, _vp, _vp, _i32, _i32, _i32,
                                   _i32, _vp, _vp]),
+    "sseqs_db_prefilter_spans": (_i32, [_vp, _vp, _vp, _i64, _i64, _vp, _i32, _vp, _vp, _i32, _i32,
+                                        _i32, _i32, _vp, _vp]),
     "sseqs_db_gather": (_i32, [_vp, _vp, _vp, _vp, _i64, _i32, _vp, _vp, _vp]),
     "sseqs_db_prefilter_kernel": (_i32, [_i32, _i32, _i32, _i32, C.c_char_p, _i32]),
     "sseqs_db_sw_affine": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp, _i32, _i32, _i32, _vp, _vp]),

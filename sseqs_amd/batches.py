@@ -11,8 +11,12 @@ A batch is *regular* when its segments are exactly those overlaps in that
 order with their reference offsets: the window starts inside a run and
 consecutive runs in it are one '@' apart.  Otherwise (a window that starts
 on '@', or '@@' inside it) the reference's ``lengths != 0`` filter shifts
-the offsets of later segments (sseqs/msa.py:253,266-267) and the batch is
-run through the ASCII path, which reproduces that shift.
+the offsets of later segments (sseqs/msa.py:253,266-267).  For such a
+*shifted* batch the plan holds the reference's own two views of its
+segments (``reference_views``): the prefilter reads segment g at the
+running sum of (length + 1) of the segments before it, and the SW stages
+take target g between delimiters g-1 and g (sseqs/sw.cu:67-77), which is a
+different run once an empty segment was dropped.
 """
 from __future__ import annotations
 
@@ -31,12 +35,62 @@ class BatchPlan:
     nseg: np.ndarray        # int64 [B] segments (non-empty) of each batch
     maxlen: np.ndarray      # int64 [B] longest segment (0 for none)
     regular: np.ndarray     # bool [B]
-    seg_off: torch.Tensor   # int64 [S] symbol offset of every segment, batch by batch
+    seg_off: torch.Tensor   # int64 [S] prefilter / gather offset of every segment, batch by batch
     seg_len: torch.Tensor   # int32 [S]
+    shifted: np.ndarray = None     # bool [B]: not regular, reference views below
+    sw_off: torch.Tensor = None    # int64 [S] SW target g of a batch (= seg_off if regular)
+    sw_len: torch.Tensor = None    # int32 [S]
 
     @property
     def n_batches(self) -> int:
         return int(self.lo.size)
+
+
+def reference_views(off: np.ndarray, ln: np.ndarray, W: int):
+    """The reference's views of one batch window of W symbols whose runs of
+    residues start at ``off`` (ascending, relative to the window) with
+    lengths ``ln``; every other symbol of the window is '@'.  Returns
+    (pf_off, sw_off, sw_len), all relative to the window:
+
+    - pf_off[g]: offset the prefilter and the hit gather use for segment g,
+      the running sum of (length + 1) of the non-empty segments before it
+      (sseqs/msa.py:253,266-267,349-350);
+    - [sw_off[g], sw_off[g] + sw_len[g]): SW target g, between delimiters
+      g-1 and g (sseqs/sw.cu:67-77 with P = delimiter positions, P[-1] =
+      -1); length 0 scores -1.  The affine stage reads the same span one
+      symbol later (P = delimiters + 1, sseqs/msa.py:331).
+
+    None when the runs do not reproduce the reference's segment lengths
+    (the batch then stays on the ASCII path)."""
+    off = np.asarray(off, dtype=np.int64)
+    ln = np.asarray(ln, dtype=np.int64)
+    m = off.size
+    gs = np.concatenate([[0], off + ln])   # gaps between runs: the '@'s
+    ge = np.concatenate([off, [W]])
+    gl = ge - gs
+    if m == 0 or (gl < 0).any():
+        return None
+    tot = int(gl.sum())
+    D = np.repeat(gs - (np.cumsum(gl) - gl), gl) + np.arange(tot, dtype=np.int64)
+    # the reference's segment lengths (sseqs/msa.py:227-253)
+    starts = D + 1
+    starts = starts[starts < W]
+    offsets = np.concatenate([starts, [W]])
+    lengths = offsets - np.concatenate([[0], offsets[:-1]])
+    lengths[:-1] -= 1
+    if tot and D[-1] == W - 1:
+        lengths[-1] -= 1
+    lengths = lengths[lengths != 0]
+    if lengths.size != m or not np.array_equal(lengths, ln):
+        return None
+    pf = np.concatenate([[0], np.cumsum(ln + 1)[:-1]])
+    g = np.arange(m)
+    prev = np.full(m, -1, dtype=np.int64)
+    if tot:
+        prev[1:] = D[np.minimum(g[1:] - 1, tot - 1)]
+    sw = prev + 1
+    stop = np.where(g < tot, D[np.minimum(g, max(tot - 1, 0))] if tot else 0, sw)
+    return pf, sw, np.maximum(stop - sw, 0)
 
 
 def plan_batches(seq_off: torch.Tensor, seq_len: torch.Tensor, n_syms: int, bs: int
@@ -109,6 +163,27 @@ def plan_batches(seq_off: torch.Tensor, seq_len: torch.Tensor, n_syms: int, bs: 
         nbad = torch.zeros(B, dtype=torch.int64, device=dev)
     regular_t = has & lead_ok & (nbad == 0)
     host = torch.stack([nseg_t, base_t, maxlen_t, regular_t.to(torch.int64)]).cpu().numpy()
-    return BatchPlan(bs=int(bs), lo=lo, n_sym=n_sym, seg_base=host[1], nseg=host[0],
-                     maxlen=host[2], regular=host[3].astype(bool), seg_off=p_off.contiguous(),
-                     seg_len=p_len.to(torch.int32).contiguous())
+    seg_off, seg_len = p_off.contiguous(), p_len.to(torch.int32).contiguous()
+    regular, nseg_h, base_h = host[3].astype(bool), host[0], host[1]
+    # shifted batches (a window starting on '@', or '@@' inside it): few, so
+    # their reference views are computed on the host, batch by batch
+    shifted = np.zeros(B, dtype=bool)
+    sw_off, sw_len = seg_off, seg_len
+    irr = np.nonzero((nseg_h > 0) & ~regular)[0]
+    if irr.size:
+        sw_off, sw_len = seg_off.clone(), seg_len.clone()
+        for b in irr:
+            lo_s, m = int(base_h[b]), int(nseg_h[b])
+            s0b = int(s0[b])
+            v = reference_views((seg_off[lo_s:lo_s + m] - s0b).cpu().numpy(),
+                                seg_len[lo_s:lo_s + m].cpu().numpy(), int(n_sym[b]))
+            if v is None:
+                continue
+            pf, so, sl = v
+            seg_off[lo_s:lo_s + m] = torch.from_numpy(s0b + pf).to(dev)
+            sw_off[lo_s:lo_s + m] = torch.from_numpy(s0b + so).to(dev)
+            sw_len[lo_s:lo_s + m] = torch.from_numpy(sl.astype(np.int32)).to(dev)
+            shifted[b] = True
+    return BatchPlan(bs=int(bs), lo=lo, n_sym=n_sym, seg_base=base_h, nseg=nseg_h,
+                     maxlen=host[2], regular=regular, seg_off=seg_off, seg_len=seg_len,
+                     shifted=shifted, sw_off=sw_off, sw_len=sw_len)

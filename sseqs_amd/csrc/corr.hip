@@ -1024,10 +1024,10 @@ struct PrefilterPlan {
     int tile = 32;    // sequences per tile
 };
 
-static PrefilterPlan plan_prefilter(int query_len, int k, int N, int mlen) {
+static PrefilterPlan plan_prefilter(int query_len, int k, int N, int mlen, bool allow_band = true) {
     PrefilterPlan P;
     const int Lqe = query_len < N ? query_len : N;
-    if (k == 4 && N >= 128 && Lqe > 0 && !getenv("SSEQS_CORR_NO_BAND")) {
+    if (allow_band && k == 4 && N >= 128 && Lqe > 0 && !getenv("SSEQS_CORR_NO_BAND")) {
         band_diagonals(N, Lqe, &P.d0, &P.rn);
         const int ND = N / 32;
         P.full = P.rn == ND;
@@ -1102,11 +1102,10 @@ extern "C" int sseqs_db_prefilter_kernel(int query_len, int k, int N, int maxlen
     return 0;
 }
 
-extern "C" int sseqs_db_prefilter(const uint8_t* packed, const int64_t* seq_off,
-                                  const int32_t* seq_len, int64_t s_begin, int64_t s_end,
-                                  const uint8_t* qrev, int query_len, const float* lutA,
-                                  const float* lutB, int vocab_size, int k, int N, int maxlen,
-                                  float* scores, hipStream_t stream) {
+static int db_prefilter(const uint8_t* packed, const int64_t* seq_off, const int32_t* seq_len,
+                        int64_t s_begin, int64_t s_end, const uint8_t* qrev, int query_len,
+                        const float* lutA, const float* lutB, int vocab_size, int k, int N,
+                        int maxlen, float* scores, hipStream_t stream, bool allow_band) {
     if (s_end < s_begin) return fail(-1, "db_prefilter: s_end < s_begin");
     const int64_t nseq = s_end - s_begin;
     if (nseq == 0) return 0;
@@ -1115,7 +1114,7 @@ extern "C" int sseqs_db_prefilter(const uint8_t* packed, const int64_t* seq_off,
     if (k < 1 || k > 20) return fail(-1, "db_prefilter: k must be 1..20");
     if (vocab_size < 1) return fail(-1, "db_prefilter: vocab_size must be >= 1");
     const int mlen = maxlen < 0 ? 0 : maxlen;
-    const PrefilterPlan P = plan_prefilter(query_len, k, N, mlen);
+    const PrefilterPlan P = plan_prefilter(query_len, k, N, mlen, allow_band);
     if (P.lds > 160 * 1024) return fail(-1, "db_prefilter: tables exceed LDS (reduce k or N)");
     const int64_t tiles = ceil_div(nseq, P.tile);
     const unsigned grid = (unsigned)(tiles < 8192 ? tiles : 8192);
@@ -1180,4 +1179,25 @@ extern "C" int sseqs_db_prefilter(const uint8_t* packed, const int64_t* seq_off,
     case 4: return go(&k_corr_db<4>);
     default: return go(&k_corr_db<8>);
     }
+}
+
+extern "C" int sseqs_db_prefilter(const uint8_t* packed, const int64_t* seq_off,
+                                  const int32_t* seq_len, int64_t s_begin, int64_t s_end,
+                                  const uint8_t* qrev, int query_len, const float* lutA,
+                                  const float* lutB, int vocab_size, int k, int N, int maxlen,
+                                  float* scores, hipStream_t stream) {
+    return db_prefilter(packed, seq_off, seq_len, s_begin, s_end, qrev, query_len, lutA, lutB,
+                        vocab_size, k, N, maxlen, scores, stream, true);
+}
+
+// Spans that may hold '@' inside them (the shifted segments of the per-batch
+// cascade): the generic kernels only, whose table gives '@' the value of
+// code 21 like the ASCII operator, while the band kernel's symbol 0 masks.
+extern "C" int sseqs_db_prefilter_spans(const uint8_t* packed, const int64_t* seq_off,
+                                        const int32_t* seq_len, int64_t s_begin, int64_t s_end,
+                                        const uint8_t* qrev, int query_len, const float* lutA,
+                                        const float* lutB, int vocab_size, int k, int N,
+                                        int maxlen, float* scores, hipStream_t stream) {
+    return db_prefilter(packed, seq_off, seq_len, s_begin, s_end, qrev, query_len, lutA, lutB,
+                        vocab_size, k, N, maxlen, scores, stream, false);
 }

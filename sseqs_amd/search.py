@@ -361,23 +361,27 @@ def _resident_batches(db: SeqDB, query: str, q: PreparedQuery, plan, top_fft: in
         for b in range(plan.n_batches):
             progress(int(plan.lo[b]), int(min(bs, db.nbytes - plan.lo[b])), float(maxl[b]))
     active = (plan.nseg > 0) & ~(np.abs(Lq - maxl.astype(np.float64)) > dlength)
-    fast = active & plan.regular
-    slow = np.nonzero(active & ~plan.regular)[0]
+    fast = active & (plan.regular | plan.shifted)
+    slow = np.nonzero(active & ~fast)[0]
     Ns = np.array([batch_corr_size(Lq, int(m)) for m in maxl], dtype=np.int64)
     fb = np.nonzero(fast)[0]
 
-    # prefilter: one launch per run of consecutive batches with one N
+    # prefilter: one launch per run of consecutive batches with one N and one
+    # kind; a shifted batch's spans may hold '@', which the generic kernels
+    # score like the reference's ASCII operator
     t0 = time.time()
     sc32 = torch.empty(plan.seg_off.numel(), dtype=torch.float32, device=dev)
     i = 0
     while i < fb.size:
         e = i
-        while e + 1 < fb.size and fb[e + 1] == fb[e] + 1 and Ns[fb[e + 1]] == Ns[fb[i]]:
+        while (e + 1 < fb.size and fb[e + 1] == fb[e] + 1 and Ns[fb[e + 1]] == Ns[fb[i]]
+               and plan.shifted[fb[e + 1]] == plan.shifted[fb[i]]):
             e += 1
         b0, b1 = fb[i], fb[e]
         s_begin = int(plan.seg_base[b0])
         s_end = int(plan.seg_base[b1] + plan.nseg[b1])
-        call("sseqs_db_prefilter", ptr(db.packed), ptr(plan.seg_off), ptr(plan.seg_len), s_begin,
+        call("sseqs_db_prefilter_spans" if plan.shifted[b0] else "sseqs_db_prefilter",
+             ptr(db.packed), ptr(plan.seg_off), ptr(plan.seg_len), s_begin,
              s_end, ptr(q.qrev), Lq, ptr(q.lut_a), ptr(q.lut_b_flat), 21, q.k, int(Ns[b0]),
              min(int(maxl[b0:b1 + 1].max()), BATCH_N_MAX), sc32.data_ptr() + 4 * s_begin, st)
         i = e + 1
@@ -397,7 +401,7 @@ def _resident_batches(db: SeqDB, query: str, q: PreparedQuery, plan, top_fft: in
     cnt1 = np.maximum(k1 - 2, 0)
     sw1 = torch.empty(cand1.numel(), dtype=torch.int32, device=dev)
     if cand1.numel():
-        call("sseqs_db_sw_linear", ptr(db.packed), ptr(plan.seg_off), ptr(plan.seg_len),
+        call("sseqs_db_sw_linear", ptr(db.packed), ptr(plan.sw_off), ptr(plan.sw_len),
              ptr(cand1), cand1.numel(), ptr(q.codes), Lq_sw, -1, ptr(sw1), st)
     # top 1/top_sw by linear score (sorted), affine SW 11/1 of segment[1:]+'@'
     k2 = cnt1 // top_sw
@@ -416,8 +420,8 @@ def _resident_batches(db: SeqDB, query: str, q: PreparedQuery, plan, top_fft: in
     n2 = int(ids2.numel())
     sw2 = torch.empty(n2, dtype=torch.int32, device=dev)
     if n2:
-        aff_off = (plan.seg_off[ids2] + 1).contiguous()
-        aff_len = plan.seg_len[ids2].contiguous()
+        aff_off = (plan.sw_off[ids2] + 1).contiguous()
+        aff_len = plan.sw_len[ids2].contiguous()
         call("sseqs_db_sw_affine", ptr(db.packed), ptr(aff_off), ptr(aff_len),
              ptr(torch.arange(n2, device=dev)), n2, ptr(q.codes), Lq_sw, 11, 1, ptr(sw2), st)
     # top top_sw_affine by affine score (sorted); the hits' fields

@@ -50,6 +50,8 @@ def test_plan_matches_oracle_segments(orc, seed, lead_at, gaps, bs, shuffle):
     assert plan.n_batches == -(-nbytes // bs)
     seg_off = plan.seg_off.numpy()
     seg_len = plan.seg_len.numpy()
+    sw_off = plan.sw_off.numpy()
+    sw_len = plan.sw_len.numpy()
     n_regular = n_irregular = 0
     for b in range(plan.n_batches):
         s0 = b * sb
@@ -64,13 +66,25 @@ def test_plan_matches_oracle_segments(orc, seed, lead_at, gaps, bs, shuffle):
                            dtype=np.int64)
         regular = lengths.size > 0 and np.array_equal(fft_off[:-1], true_st)
         assert bool(plan.regular[b]) == regular
+        sl = slice(plan.seg_base[b], plan.seg_base[b] + plan.nseg[b])
         if regular:
             n_regular += 1
-            sl = slice(plan.seg_base[b], plan.seg_base[b] + plan.nseg[b])
             assert np.array_equal(seg_off[sl] - s0, fft_off[:-1])
             assert np.array_equal(seg_len[sl], lengths)
+            assert np.array_equal(sw_off[sl], seg_off[sl]) and np.array_equal(sw_len[sl],
+                                                                              seg_len[sl])
         elif lengths.size:
             n_irregular += 1
+            # the shifted batch's two reference views
+            assert plan.shifted[b]
+            assert np.array_equal(seg_off[sl] - s0, fft_off[:-1])
+            assert np.array_equal(seg_len[sl], lengths)
+            # SW target g between delimiters g-1 and g (P[-1] = -1), for every
+            # g the selection can reach (the first and last are dropped)
+            for g in range(1, lengths.size - 1):
+                st = (D[g - 1] if g > 0 else -1) + 1
+                assert sw_off[sl][g] - s0 == st, (b, g)
+                assert sw_len[sl][g] == D[g] - st, (b, g)
     assert n_regular > 0
     if lead_at or gaps:
         assert n_irregular > 0

@@ -215,7 +215,10 @@ def test_batch_cascade_shifted_segments(orc, dev):
     packed = torch.from_numpy(orc.xbit_pack(a)).to(dev)
     db = SeqDB.from_packed(packed, a.size)
     plan = db.batch_plan(bs)
-    assert (~plan.regular & (plan.nseg > 0)).any() and plan.regular.any()
+    irregular = ~plan.regular & (plan.nseg > 0)
+    assert irregular.any() and plan.regular.any()
+    # every such batch gets the reference's views and stays on the resident path
+    assert plan.shifted[irregular].all()
     kw = dict(top_fft=10, top_sw=3, top_sw_affine=2)
     got_r = _check_batch_cascade(orc, db, a, query, bs, True, **kw)
     got_a = _check_batch_cascade(orc, db, a, query, bs, False, **kw)
