@@ -210,10 +210,16 @@ def shuffled_copy(db, seed: int = 3):
     perm = torch.randperm(db.n, device=dev, generator=g)
     ln = db.seq_len[perm].to(torch.int64)
     out_off = torch.cumsum(ln + 1, 0) - (ln + 1)
+    src_off = db.seq_off[perm]
     total = int((ln + 1).sum().item())
-    seg = torch.repeat_interleave(torch.arange(db.n, device=dev), ln)
-    pos = torch.arange(seg.numel(), device=dev) - (torch.cumsum(ln, 0) - ln)[seg]
     out = torch.full((total,), 64, dtype=torch.uint8, device=dev)
-    out[out_off[seg] + pos] = asc[db.seq_off[perm][seg] + pos]
-    del seg, pos, asc
+    # a million sequences per step: each index array stays far below 2^31
+    # elements, which whole-database gathers and scatters exceed
+    for c0 in range(0, db.n, 1 << 20):
+        lc = ln[c0:c0 + (1 << 20)]
+        seg = torch.repeat_interleave(torch.arange(lc.numel(), device=dev), lc)
+        pos = torch.arange(seg.numel(), device=dev) - (torch.cumsum(lc, 0) - lc)[seg]
+        out[out_off[c0:c0 + (1 << 20)][seg] + pos] = asc[src_off[c0:c0 + (1 << 20)][seg] + pos]
+        del seg, pos
+    del asc
     return SeqDB.from_packed(ops.compress(out), total), perm
