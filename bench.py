@@ -196,8 +196,8 @@ def main() -> int:
     ap.add_argument("--stream-only", action="store_true",
                     help="time only the unpack, pack and index kernels (for traces)")
     ap.add_argument("--shuffled", action="store_true",
-                    help="time the q_len=200 prefilter on the database and on a copy of its stream in "
-                         "random order (use with --n-seqs)")
+                    help="time the q_len=200 prefilter and per-batch cascade on the database and on a "
+                         "copy of its stream in random order (use with --n-seqs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="run a plain device copy of BYTES bytes and exit (counter calibration)")
     args = ap.parse_args()
@@ -228,24 +228,38 @@ def main() -> int:
     log(f"generated {db.n} sequences, {db.residues} residues, {db.nbytes / 1e9:.2f} GB packed "
         f"in {time.time() - t0:.1f} s")
     if args.shuffled:
-        sh, perm = synth.shuffled_copy(db)
+        # the length-sorted database first, then a copy of its stream in random
+        # order (built only after, so that its memory does not weigh on the first)
+        from sseqs_amd.search import batch_search
         q = prepare(q200, device=dev)
-        res = {}
-        for name, d in (("length_sorted_stream", db), ("shuffled_stream", sh)):
-            s = prefilter_scores(d, q)
+
+        def timed(fn):
+            fn()
             torch.cuda.synchronize()
             t = time.perf_counter()
             for _ in range(args.steps):
-                s = prefilter_scores(d, q)
+                out = fn()
             torch.cuda.synchronize()
-            ms = (time.perf_counter() - t) / args.steps * 1e3
-            res[name] = {"ms": round(ms, 3), "db_gb_per_s": round(d.nbytes / ms / 1e6, 1)}
-            res[name + "_scores"] = s
-        back = perm[sh.orig_id] if sh.orig_id is not None else perm
-        same = torch.equal(res.pop("length_sorted_stream_scores")[back],
-                           res.pop("shuffled_stream_scores"))
-        print(json.dumps({"prefilter_q200_shuffled": res, "n_seqs": db.n,
-                          "scores_identical": bool(same)}), flush=True)
+            return (time.perf_counter() - t) / args.steps * 1e3, out
+
+        res, scores = {}, {}
+        for name in ("length_sorted_stream", "shuffled_stream"):
+            if name == "shuffled_stream":
+                d, perm = synth.shuffled_copy(db)
+                torch.cuda.empty_cache()
+            else:
+                d = db
+            ms, scores[name] = timed(lambda: prefilter_scores(d, q))
+            scores[name] = scores[name].clone()
+            bms, _ = timed(lambda: batch_search(d, q200, q, bs=50_000_000))
+            res[name] = {"prefilter_ms": round(ms, 3), "db_gb_per_s": round(d.nbytes / ms / 1e6, 1),
+                         "batch_cascade_ms": round(bms, 3),
+                         "batch_prefilter_order": "length-sorted per batch"
+                         if d.batch_plan(50_000_000).pf_perm is not None else "stream"}
+        back = perm[d.orig_id] if d.orig_id is not None else perm
+        same = torch.equal(scores["length_sorted_stream"][back], scores["shuffled_stream"])
+        print(json.dumps({"shuffled": res, "q_len": 200, "n_seqs": db.n,
+                          "prefilter_scores_identical": bool(same)}), flush=True)
         return 0
     if args.stream_only:
         print(json.dumps({"stream_kernels": kernel_rates(db, dev)}), flush=True)

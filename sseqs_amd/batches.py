@@ -40,6 +40,13 @@ class BatchPlan:
     shifted: np.ndarray = None     # bool [B]: not regular, reference views below
     sw_off: torch.Tensor = None    # int64 [S] SW target g of a batch (= seg_off if regular)
     sw_len: torch.Tensor = None    # int32 [S]
+    # the prefilter's order when the stream is not in length order: each
+    # batch's segments by length (pf_perm[i] = plan index of the i-th), so
+    # that a tile of 16 does not run to the length of its longest member;
+    # None when the plan order wastes little
+    pf_perm: torch.Tensor = None   # int64 [S]
+    pf_off: torch.Tensor = None    # int64 [S] seg_off[pf_perm]
+    pf_len: torch.Tensor = None    # int32 [S] seg_len[pf_perm]
 
     @property
     def n_batches(self) -> int:
@@ -184,6 +191,25 @@ def plan_batches(seq_off: torch.Tensor, seq_len: torch.Tensor, n_syms: int, bs: 
             sw_off[lo_s:lo_s + m] = torch.from_numpy(s0b + so).to(dev)
             sw_len[lo_s:lo_s + m] = torch.from_numpy(sl.astype(np.int32)).to(dev)
             shifted[b] = True
-    return BatchPlan(bs=int(bs), lo=lo, n_sym=n_sym, seg_base=base_h, nseg=nseg_h,
+    plan = BatchPlan(bs=int(bs), lo=lo, n_sym=n_sym, seg_base=base_h, nseg=nseg_h,
                      maxlen=host[2], regular=regular, seg_off=seg_off, seg_len=seg_len,
                      shifted=shifted, sw_off=sw_off, sw_len=sw_len)
+    if total:
+        # prefilter tiles hold 16 consecutive segments and run to the longest;
+        # sort each batch by length when that wastes more than 10 %
+        key = (p_b << 32) | p_len
+        perm = torch.sort(key, stable=True).indices
+        if _tile_cost(p_len) > 1.1 * _tile_cost(p_len[perm]):
+            plan.pf_perm = perm
+            plan.pf_off = seg_off[perm].contiguous()
+            plan.pf_len = seg_len[perm].contiguous()
+        del key, perm
+    return plan
+
+
+def _tile_cost(lens: torch.Tensor, tile: int = 16) -> float:
+    """Sum over consecutive groups of ``tile`` of the group's longest length."""
+    n = lens.numel()
+    pad = (-n) % tile
+    x = torch.cat([lens, lens.new_zeros(pad)]) if pad else lens
+    return float(x.view(-1, tile).max(dim=1).values.sum().item())

@@ -371,6 +371,11 @@ def _resident_batches(db: SeqDB, query: str, q: PreparedQuery, plan, top_fft: in
     # score like the reference's ASCII operator
     t0 = time.time()
     sc32 = torch.empty(plan.seg_off.numel(), dtype=torch.float32, device=dev)
+    # each batch's segments in length order when the plan has one (same
+    # per-batch ranges), scores put back in plan order after the launches
+    pf_off, pf_len = ((plan.pf_off, plan.pf_len) if plan.pf_perm is not None
+                      else (plan.seg_off, plan.seg_len))
+    pf_out = torch.empty_like(sc32) if plan.pf_perm is not None else sc32
     i = 0
     while i < fb.size:
         e = i
@@ -381,10 +386,13 @@ def _resident_batches(db: SeqDB, query: str, q: PreparedQuery, plan, top_fft: in
         s_begin = int(plan.seg_base[b0])
         s_end = int(plan.seg_base[b1] + plan.nseg[b1])
         call("sseqs_db_prefilter_spans" if plan.shifted[b0] else "sseqs_db_prefilter",
-             ptr(db.packed), ptr(plan.seg_off), ptr(plan.seg_len), s_begin,
+             ptr(db.packed), ptr(pf_off), ptr(pf_len), s_begin,
              s_end, ptr(q.qrev), Lq, ptr(q.lut_a), ptr(q.lut_b_flat), 21, q.k, int(Ns[b0]),
-             min(int(maxl[b0:b1 + 1].max()), BATCH_N_MAX), sc32.data_ptr() + 4 * s_begin, st)
+             min(int(maxl[b0:b1 + 1].max()), BATCH_N_MAX), pf_out.data_ptr() + 4 * s_begin, st)
         i = e + 1
+    if plan.pf_perm is not None:
+        sc32[plan.pf_perm] = pf_out
+        del pf_out
     sc16 = sc32.half()
     del sc32
     if on_scores is not None:

@@ -88,6 +88,28 @@ def test_plan_matches_oracle_segments(orc, seed, lead_at, gaps, bs, shuffle):
     assert n_regular > 0
     if lead_at or gaps:
         assert n_irregular > 0
+    # the prefilter's order: each batch's own segments, by length, stable
+    if bs >= 1000:  # ~50 whole runs of 1..59 residues per batch: sorting them pays
+        assert plan.pf_perm is not None
+    if plan.pf_perm is None:
+        return
+    perm = plan.pf_perm.numpy()
+    for b in range(plan.n_batches):
+        lo_s, hi_s = int(plan.seg_base[b]), int(plan.seg_base[b] + plan.nseg[b])
+        pb = perm[lo_s:hi_s]
+        assert np.array_equal(np.sort(pb), np.arange(lo_s, hi_s))
+        assert np.array_equal(pb, lo_s + np.argsort(seg_len[lo_s:hi_s], kind="stable"))
+    assert np.array_equal(plan.pf_off.numpy(), seg_off[perm])
+    assert np.array_equal(plan.pf_len.numpy(), seg_len[perm])
+
+
+def test_plan_keeps_length_sorted_order():
+    """A stream already in length order gets no prefilter permutation."""
+    ln = np.repeat(np.arange(10, 200), 40).astype(np.int64)
+    off = np.concatenate([[0], np.cumsum(ln + 1)[:-1]])
+    n = int(off[-1] + ln[-1] + 1)
+    plan = plan_batches(torch.from_numpy(off), torch.from_numpy(ln.astype(np.int32)), n, 500)
+    assert plan.pf_perm is None
 
 
 def test_plan_rejects_unaligned_batches():
