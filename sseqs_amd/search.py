@@ -409,8 +409,17 @@ def _resident_batches(db: SeqDB, query: str, q: PreparedQuery, plan, top_fft: in
     cnt1 = np.maximum(k1 - 2, 0)
     sw1 = torch.empty(cand1.numel(), dtype=torch.int32, device=dev)
     if cand1.numel():
+        # a wave scores four targets to the length of the longest: in a file
+        # not in length order (the plan then has pf_perm) score them by length
+        order1 = (torch.sort(plan.sw_len[cand1], stable=True).indices
+                  if plan.pf_perm is not None else None)
+        c_sw = cand1 if order1 is None else cand1[order1]
+        out1 = sw1 if order1 is None else torch.empty_like(sw1)
         call("sseqs_db_sw_linear", ptr(db.packed), ptr(plan.sw_off), ptr(plan.sw_len),
-             ptr(cand1), cand1.numel(), ptr(q.codes), Lq_sw, -1, ptr(sw1), st)
+             ptr(c_sw), c_sw.numel(), ptr(q.codes), Lq_sw, -1, ptr(out1), st)
+        if order1 is not None:
+            sw1[order1] = out1
+            del out1, c_sw
     # top 1/top_sw by linear score (sorted), affine SW 11/1 of segment[1:]+'@'
     k2 = cnt1 // top_sw
     ok = (cnt1 > 0) & (k2 >= top_sw_affine)
