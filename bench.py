@@ -195,6 +195,9 @@ def main() -> int:
                     help="run only the q_len=200 prefilter sweep --steps times (for counters)")
     ap.add_argument("--stream-only", action="store_true",
                     help="time only the unpack, pack and index kernels (for traces)")
+    ap.add_argument("--sw-only", action="store_true",
+                    help="time only affine SW (11/1) of the first 20 M sequences at q_len=200 "
+                         "(for counters)")
     ap.add_argument("--shuffled", action="store_true",
                     help="time the q_len=200 prefilter and per-batch cascade on the database and on a "
                          "copy of its stream in random order (use with --n-seqs)")
@@ -227,6 +230,22 @@ def main() -> int:
     torch.cuda.synchronize()
     log(f"generated {db.n} sequences, {db.residues} residues, {db.nbytes / 1e9:.2f} GB packed "
         f"in {time.time() - t0:.1f} s")
+    if args.sw_only:
+        from sseqs_amd.search import sw_affine_ids
+        q = prepare(q200, device=dev)
+        ids = torch.arange(min(db.n, 20_000_000), device=dev)
+        cells = len(q200) * int(db.seq_len[:ids.numel()].to(torch.int64).sum().item())
+        sw_affine_ids(db, q, ids)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            sw_affine_ids(db, q, ids)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t) / args.steps * 1e3
+        print(json.dumps({"sw_affine_q200": {"targets": ids.numel(), "cells": cells,
+                                             "ms": round(ms, 3),
+                                             "gcups": round(cells / ms / 1e6, 1)}}), flush=True)
+        return 0
     if args.shuffled:
         # the length-sorted database first, then a copy of its stream in random
         # order (built only after, so that its memory does not weigh on the first)
