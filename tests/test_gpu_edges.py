@@ -92,3 +92,40 @@ def test_batch_cascade_query_without_residues(orc, dev, tmp_path):
                   top_sw=4)
     hits = rank_hits(got, 64)
     assert out.read_text() == orc.a3m(query, [orc.backtrack(query, h.seq) for h in hits])
+
+
+def test_msa_call_options(dev, tmp_path, monkeypatch):
+    """msa()'s other call forms (sseqs/msa.py:71-115): a list of queries with
+    a list of files gives each query's single-call text; blocking=False
+    returns a thread whose file matches; filenames=None writes
+    sha256(query)[:8] + '.a3m'; save_raw keeps the raw hits."""
+    import hashlib
+    import pickle
+    import threading
+
+    import sseqs_amd
+    from sseqs_amd import synth
+    from sseqs_amd.db import SeqDB
+    announce("msa call options")
+    qs = [synth.make_query(90, 0, seed=5), synth.make_query(60, 1, seed=5)]
+    db = SeqDB.synthetic(6000, qs, seed=6, device=dev)
+    kw = dict(num_msas=32, verbose=False, db=db, bs=200_000, top_fft=20, top_sw=4)
+    single = []
+    for i, q in enumerate(qs):
+        fn = tmp_path / f"single{i}.a3m"
+        sseqs_amd.msa(q, str(fn), **kw)
+        single.append(fn.read_text())
+    pair = [tmp_path / "p0.a3m", tmp_path / "p1.a3m"]
+    sseqs_amd.msa(qs, [str(p) for p in pair], **kw)
+    assert [p.read_text() for p in pair] == single
+    t = sseqs_amd.msa(qs[0], str(tmp_path / "bg.a3m"), blocking=False, **kw)
+    assert isinstance(t, threading.Thread)
+    t.join()
+    assert (tmp_path / "bg.a3m").read_text() == single[0]
+    monkeypatch.chdir(tmp_path)
+    sseqs_amd.msa(qs[1], save_raw=str(tmp_path / "raw.pkl"), **kw)
+    assert (tmp_path / (hashlib.sha256(qs[1].encode()).hexdigest()[:8] + ".a3m")).read_text() \
+        == single[1]
+    with open(tmp_path / "raw.pkl", "rb") as fh:  # written by msa() above
+        raw = pickle.load(fh)
+    assert raw[0] == qs[1] and len(raw[1]) == len(raw[2]) == len(raw[4]) > 0
