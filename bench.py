@@ -489,22 +489,31 @@ def main() -> int:
     if args.batch_only:
         measure_batch(q200)
         return 0
+    def side(name, fn):
+        """An auxiliary measurement: a failure is logged and reported as null
+        rather than taking the headline line with it."""
+        try:
+            return fn()
+        except Exception as e:  # noqa: BLE001 - reported, not hidden
+            log(f"{name} failed: {type(e).__name__}: {e}")
+            return None
+
     r200, launches200 = measure(q200)
-    rbatch = measure_batch(q200) if args.batch_steps > 0 else None
-    re2e = measure_msa(q200) if args.batch_steps > 0 else None
-    rates = kernel_rates(db, dev)
+    rbatch = side("batch cascade", lambda: measure_batch(q200)) if args.batch_steps > 0 else None
+    re2e = side("msa call", lambda: measure_msa(q200)) if args.batch_steps > 0 else None
+    rates = side("stream kernels", lambda: kernel_rates(db, dev))
     log(f"stream kernels {json.dumps(rates)}")
-    recalls = [recall(q200)] if not args.no_recall else []
+    recalls = [side("recall q_len 200", lambda: recall(q200))] if not args.no_recall else []
     r500 = r500l = None
     if not args.no_q500:
         r500, _ = measure(q500)
         r500l, _ = measure(q500, linear_gap=-4)
         if not args.no_recall:
-            recalls.append(recall(q500))
+            recalls.append(side("recall q_len 500", lambda: recall(q500)))
     cpu = None
     if not args.no_cpu:
         t0 = time.time()
-        cpu = cpu_baseline(db, q200)
+        cpu = side("cpu baseline", lambda: cpu_baseline(db, q200))
         log(f"cpu baseline {cpu} in {time.time() - t0:.1f} s")
 
     # roofline of the dominant kernel launch of the q_len=200 step
