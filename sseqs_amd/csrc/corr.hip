@@ -691,9 +691,14 @@ __global__ __launch_bounds__(64, (G == 1 && R <= 8 ? 2 : 1)) void k_corr_band(
             L[gg] = len[s_begin + sid];
         }
     };
+    // A tile is staged either as one byte range (its sequences lie close
+    // together: a length-sorted stream) or, when that range exceeds the
+    // stage, as TS slots of scb 16-byte chunks, slot 16gg+col holding the
+    // sequence of column col of group gg from its 16-byte-aligned first
+    // byte (sequences scattered over the stream); scb = 0 for a range.
     struct Tile {
         int64_t o[G], b_lo, nchunk;
-        int Le[G], C;
+        int Le[G], C, scb;
         bool in[G], fast;
     };
     auto info = [&](int64_t t, const int64_t* o, const int* L) -> Tile {
@@ -748,6 +753,19 @@ __global__ __launch_bounds__(64, (G == 1 && R <= 8 ? 2 : 1)) void k_corr_band(
         // copy only bytes holding residues in use: at most 24 bytes past the
         // last residue of the database are read
         T.nchunk = T.fast ? ((((ehi * 5) >> 3) + 8 - T.b_lo + 15) >> 4) : 0;
+        T.scb = 0;
+        if (!T.fast) {
+            // slots: a sequence's copied bytes are at most 15 (alignment)
+            // + 5*kp/8 + 1 + 8; its windows end at most 20C+26 bytes into
+            // its slot, and the last slot's must end inside the stage
+            const int scb = (((5 * kp) >> 3) + 24 + 15) >> 4;
+            const int used = max(TS * 16 * scb, (TS - 1) * 16 * scb + 20 * T.C + 26);
+            if (used <= CAPB) {
+                T.scb = scb;
+                T.fast = true;
+                T.nchunk = TS * scb;
+            }
+        }
         return T;
     };
     // One global_load_lds_dwordx4 copies 64 x 16 bytes: lane l's 16 bytes go
@@ -756,14 +774,43 @@ __global__ __launch_bounds__(64, (G == 1 && R <= 8 ? 2 : 1)) void k_corr_band(
     auto issue = [&](const Tile& T, uint8_t* buf) {
         const uint32_t lds0 =
             (uint32_t)(uintptr_t)(__attribute__((address_space(3))) uint8_t*)buf;
-        for (int64_t c0 = 0; c0 < T.nchunk; c0 += 64)
-            if (c0 + lane < T.nchunk) {
-                const uint8_t* src = packed + T.b_lo + 16 * (c0 + lane);
+        if (T.scb == 0) {
+            for (int64_t c0 = 0; c0 < T.nchunk; c0 += 64)
+                if (c0 + lane < T.nchunk) {
+                    const uint8_t* src = packed + T.b_lo + 16 * (c0 + lane);
+                    asm volatile("global_load_lds_dwordx4 %1, off"
+                                 :
+                                 : "{m0}"(lds0 + 16u * (uint32_t)c0), "v"(src)
+                                 : "memory");
+                }
+            return;
+        }
+        // slots: chunk c is chunk j of slot s; the slot's sequence comes
+        // from lane s & 15 (every row holds the same column)
+        for (int c0 = 0; c0 < (int)T.nchunk; c0 += 64) {
+            const int c = c0 + lane;
+            const int s = c / T.scb, j = c - s * T.scb;
+            int64_t os = 0;
+            int ls = 0;
+#pragma unroll
+            for (int gg = 0; gg < G; ++gg) {
+                const int64_t v = __shfl(T.o[gg], s & 15);
+                const int l = __shfl(T.Le[gg], s & 15);
+                if ((s >> 4) == gg) {
+                    os = v;
+                    ls = l;
+                }
+            }
+            const int64_t bb = ((os * 5) >> 3) & ~(int64_t)15;
+            const int64_t nb = (((os + ls) * 5) >> 3) + 8 - bb;  // bytes holding its residues
+            if (c < (int)T.nchunk && ls > 0 && 16 * (int64_t)j < nb) {
+                const uint8_t* src = packed + bb + 16 * j;
                 asm volatile("global_load_lds_dwordx4 %1, off"
                              :
                              : "{m0}"(lds0 + 16u * (uint32_t)c0), "v"(src)
                              : "memory");
             }
+        }
     };
 
     int64_t o_n[G], o_c[G];
@@ -803,7 +850,10 @@ __global__ __launch_bounds__(64, (G == 1 && R <= 8 ? 2 : 1)) void k_corr_band(
         const int C = cur.C;
         int64_t relbit[G];  // bit of each sequence of this lane in the stage
 #pragma unroll
-        for (int gg = 0; gg < G; ++gg) relbit[gg] = 5 * cur.o[gg] - 8 * cur.b_lo;
+        for (int gg = 0; gg < G; ++gg)
+            relbit[gg] = cur.scb ? 128 * (int64_t)cur.scb * (16 * gg + col) + 5 * cur.o[gg] -
+                                       8 * (((cur.o[gg] * 5) >> 3) & ~(int64_t)15)
+                                 : 5 * cur.o[gg] - 8 * cur.b_lo;
 
         // B fragments (4 sub-chunks) of chunk v of group gg: this lane's 8
         // positions 32v+8g.., symbol 0 past the sequence's end
@@ -1045,18 +1095,26 @@ static PrefilterPlan plan_prefilter(int query_len, int k, int N, int mlen, bool 
             P.tile = 16 * P.G;
             // a tile of length-sorted sequences spans about that many of them
             const int span = (min(mlen, N) + 31) & ~31;
-            P.capb = (int)((((int64_t)P.tile * (span + 1) * 5) / 8 + 48 + 15) & ~15);
+            const int capb = (int)((((int64_t)P.tile * (span + 1) * 5) / 8 + 48 + 15) & ~15);
+            // a tile of scattered sequences of up to span symbols, one
+            // 16-byte-aligned slot each (the kernel's slot layout)
+            const int scb = (((5 * span) >> 3) + 24 + 15) >> 4;
+            const int slots = (max(P.tile * 16 * scb,
+                                   (P.tile - 1) * 16 * scb + 20 * (span / 32) + 26) + 15) & ~15;
             const size_t qt = (size_t)4 * (N + 8) * 2;
-            const size_t sbytes = (size_t)((P.capb + 64 + 15) & ~15);
             const size_t save = P.full ? 0 : (size_t)P.G * P.R * 64 * 16;
-            auto lds_for = [&](int nbuf) {
-                const size_t run = save + nbuf * sbytes;
+            auto lds_for = [&](int cap, int nbuf) {
+                const size_t run = save + nbuf * (size_t)((cap + 64 + 15) & ~15);
                 return ((size_t)CORR_NB * 8 + (qt > run ? qt : run) + 15) & ~(size_t)15;
             };
             // two stage buffers while the blocks of 8 waves per CU still fit
-            P.dbl = lds_for(2) <= (P.G == 1 ? 20 : 40) * 1024;
+            const size_t budget = (size_t)(P.G == 1 ? 20 : 40) * 1024;
+            P.dbl = lds_for(capb, 2) <= budget;
             if (const char* e = getenv("SSEQS_CORR_DBL")) P.dbl = atoi(e) != 0;
-            P.lds = lds_for(P.dbl ? 2 : 1);
+            const int nbuf = P.dbl ? 2 : 1;
+            // room for the slot layout where that budget allows it
+            P.capb = slots > capb && lds_for(slots, nbuf) <= budget ? slots : capb;
+            P.lds = lds_for(P.capb, nbuf);
             return P;
         }
     }

@@ -350,8 +350,9 @@ def test_db_prefilter_sequences_longer_than_2048(orc, dev):
 @pytest.mark.parametrize("Lq,k", [(50, 4), (200, 4), (500, 4), (200, 1), (90, 3)])
 def test_db_prefilter_any_index_order(orc, dev, Lq, k):
     """Resident-database prefilter on an index that is length-sorted but not
-    in stream order (a file that was not sorted), which takes the kernel's
-    global-memory path, and on the same data in stream order."""
+    in stream order (a file that was not sorted), whose tiles the band
+    kernel stages into per-sequence slots (or, past the stage, decodes from
+    global memory), and on the same data in stream order."""
     from sseqs_amd.db import SeqDB
     from sseqs_amd.query import prepare
     from sseqs_amd.search import prefilter_scores
