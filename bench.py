@@ -223,10 +223,8 @@ def main() -> int:
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    q200 = synth.make_query(200, 0, seed=1)
-    q500 = synth.make_query(500, 1, seed=1)
     t0 = time.time()
-    db = SeqDB.synthetic(args.n_seqs, [q200, q500], seed=0, device=dev)
+    db, q200, q500, batch_queries = synth.bench_db(args.n_seqs, device=dev)
     torch.cuda.synchronize()
     log(f"generated {db.n} sequences, {db.residues} residues, {db.nbytes / 1e9:.2f} GB packed "
         f"in {time.time() - t0:.1f} s")

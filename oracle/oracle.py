@@ -245,6 +245,22 @@ def corr(qrev, lutA: np.ndarray, lutB: np.ndarray, k: int, flat, offsets, length
     return out
 
 
+def corr_tol(qrev, lutA: np.ndarray, lutB: np.ndarray, k: int, flat, offsets, lengths,
+             maxlen: int, N: int, x64: np.ndarray | None = None) -> np.ndarray:
+    """Per-sequence tolerance of a device prefilter score against the FP64
+    closed form: 1e-2 * |x64| plus the error that rounding the two LUT
+    operands to fp16 (relative 2^-11 each, so 2^-10 + 2^-22 per product) and
+    summing k * Lq products in fp32 (at most k * Lq * 2^-24 relative to the
+    absolute sum) can cause.  Both scale with the sum of |a_r| * |b_r| over
+    the terms of a shift, which is at most the closed form evaluated with
+    |LUT_A| and |LUT_B| (the maximum over shifts of those absolute sums)."""
+    if x64 is None:
+        x64 = corr(qrev, lutA, lutB, k, flat, offsets, lengths, maxlen, N)
+    absum = corr(qrev, np.abs(lutA), np.abs(lutB), k, flat, offsets, lengths, maxlen, N)
+    n_terms = int(k) * len(qrev)
+    return 1e-2 * np.abs(x64) + (2.0**-10 + 2.0**-22 + n_terms * 2.0**-24) * absum
+
+
 def corr_fft_emulation(qrev, lutA, lutB, k: int, seg_codes_fn, L: int, maxlen: int, N: int) -> float:
     """The reference's own transform pipeline (packed real FFT, frequency
     product, packed inverse, 1/N, even samples) evaluated in FP64 with
@@ -366,23 +382,25 @@ def nextpow2(x: int) -> int:
 
 
 def batch_cascade(db_ascii: np.ndarray, query: str, bs_syms: int, top_fft=200, top_sw=10,
-                  top_sw_affine=2, fft_rank=4, fft_scores=None):
+                  top_sw_affine=2, fft_rank=4, fft_scores=None, with_tol=False):
     """Per-batch cascade of sseqs/msa.py:217-370 on a decompressed stream cut
     into batches of bs_syms symbols.  fft_scores(j) may supply the batch's
     prefilter scores (the stages after it are integer and exact); otherwise
     the FP64 closed form rounded to fp16 is used.  Returns the hits as
     (norm float32, affine, linear, seq) in collection order, plus the
-    batch's FP64 prefilter scores."""
+    batch's FP64 prefilter scores (and, with_tol, their corr_tol
+    tolerances)."""
     A, Bt = weighted_luts(query, fft_rank)
     k = max(1, min(fft_rank, A.shape[0]))
     qrev, qc = encode_query_rev(query), encode_query_sw(query)
     Lq = len(query)
-    hits, fft64 = [], []
+    hits, fft64, tol64 = [], [], []
     for j, lo in enumerate(range(0, db_ascii.size, bs_syms)):
         a = db_ascii[lo: lo + bs_syms]
         D, starts, lengths, off = segments(a)
         if lengths.size == 0:
             fft64.append(np.zeros(0))
+            tol64.append(np.zeros(0))
             continue
         maxlen = int(lengths.max())
         # sseqs/msa.py:261; sizes outside 16..2048 (scored 0 by the
@@ -391,6 +409,8 @@ def batch_cascade(db_ascii: np.ndarray, query: str, bs_syms: int, top_fft=200, t
         n = min(max(nextpow2(max(Lq, maxlen)), 16), 2048)
         s64 = corr(qrev, A, Bt, k, a, off[:-1], lengths, maxlen, n)
         fft64.append(s64)
+        if with_tol:
+            tol64.append(corr_tol(qrev, A, Bt, k, a, off[:-1], lengths, maxlen, n, s64))
         s = fft_scores(j) if fft_scores is not None else s64.astype(np.float16)
         good = np.sort(topk_desc(np.asarray(s, np.float64), s.size // top_fft))[1:-1]
         if good.size == 0:
@@ -409,6 +429,8 @@ def batch_cascade(db_ascii: np.ndarray, query: str, bs_syms: int, top_fft=200, t
             st, L = int(off[seg]), int(lengths[seg])
             hits.append((np.float32(norm[r]), int(gap[t2][r]), int(lin[p]),
                          a[st: st + min(L, 500)].tobytes().decode("latin-1")))
+    if with_tol:
+        return hits, fft64, tol64
     return hits, fft64
 
 

@@ -213,20 +213,40 @@ class SeqDB:
 _default_db: SeqDB | None = None
 
 
+def env_chunks() -> int | None:
+    """CHUNKS as the reference reads it (sseqs/msa.py:14): the number of 1 GB
+    chunks of the database to load, 4 when unset and at least 1.
+    ``CHUNKS=all`` (an addition) loads the whole file and returns None."""
+    v = os.environ.get("CHUNKS", "4").strip()
+    if v.lower() == "all":
+        return None
+    c = int(v)
+    if c <= 0:
+        raise ValueError(f"CHUNKS must be a positive number of 1 GB chunks or 'all', got {v!r}")
+    return c
+
+
+def chunks_tag() -> str:
+    """The database-size part of the server's cache folder names
+    (server.py:25,72): CHUNKS as loaded by default_db."""
+    c = env_chunks()
+    return "all" if c is None else str(c)
+
+
 def default_db(device="cuda") -> SeqDB:
     """Database named by DBPATH (default uniref_bfd_mgy_cf.xbit), first CHUNKS
-    GB (sseqs/msa.py:14,18), followed by the antibody database named by AB
-    when set, loaded on first use."""
+    GB (sseqs/msa.py:14,18; 4 when unset), followed by the antibody database
+    named by AB when set, loaded on first use."""
     global _default_db
     if _default_db is None:
         path = os.environ.get("DBPATH", "uniref_bfd_mgy_cf.xbit")
         if not os.path.exists(path):
             raise FileNotFoundError(
                 f"Didn't find MSA database at `{path}` (set DBPATH, or pass db=...)")
-        chunks = os.environ.get("CHUNKS")
+        chunks = env_chunks()
         ab = os.environ.get("AB") or None  # sseqs/msa.py:39-40,57-62
-        _default_db = SeqDB.from_xbit(path, int(chunks) if chunks else None, device=device,
-                                      ab_path=ab)
+        _default_db = SeqDB.from_xbit(path, chunks, device=device, ab_path=ab)
+        _default_db.extra["chunks_tag"] = chunks_tag()
     return _default_db
 
 

@@ -187,10 +187,33 @@ def topk_indices(scores: torch.Tensor, k: int, sorted: bool = True) -> torch.Ten
     out = torch.empty(k, dtype=torch.int64, device=s.device)
     if k == 0:
         return out
+    big = sorted and k > SORTED_MAX
     ws = _lib.workspace(_lib.load().sseqs_topk_ws_bytes(n), s.device)
-    call("sseqs_topk", ptr(s), _DT[s.dtype], n, k, int(bool(sorted)), ptr(out), ptr(ws),
-         ws.numel(), stream_of(s.device))
+    call("sseqs_topk", ptr(s), _DT[s.dtype], n, k, int(bool(sorted) and not big), ptr(out),
+         ptr(ws), ws.numel(), stream_of(s.device))
+    if big:
+        out = _order_desc(s, out)
     return out
+
+
+# sseqs_topk orders its output in LDS up to this k; larger sorted selections
+# are ordered after an unsorted select
+SORTED_MAX = 16384
+
+
+def _order_desc(s: torch.Tensor, idx: torch.Tensor) -> torch.Tensor:
+    """Order selected indices ``idx`` (ascending) by (score desc, index asc),
+    NaN last, as sseqs_topk's sorted output: one stable descending sort of
+    the selected scores keeps ascending index order among equal scores."""
+    v = s[idx]
+    if v.is_floating_point():
+        nan = torch.isnan(v)
+        if bool(nan.any()):
+            keep = idx[~nan]
+            order = torch.sort(v[~nan], descending=True, stable=True).indices
+            return torch.cat([keep[order], idx[nan]])
+    order = torch.sort(v, descending=True, stable=True).indices
+    return idx[order]
 
 
 def topk_segments(scores: torch.Tensor, begin, n, k, *, sorted: bool = False,

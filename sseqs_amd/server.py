@@ -25,7 +25,8 @@ Differences from server.py:
   reference starts uvicorn at import.
 * Searches run in a worker thread, one at a time, so the event loop keeps
   serving downloads meanwhile. The reference called ``msa()`` inside the
-  event loop.
+  event loop. Each a3m is written under a temporary name and renamed into
+  the cache, and the cache is checked again under the search lock.
 * ``/msa`` passes the file names as ``filenames``, the name ``msa()``
   takes. The reference passed ``filenames_a3m=`` (server.py:147), so its
   ``/msa`` always answered with the error line.
@@ -40,6 +41,7 @@ import json
 import os
 import tarfile
 import threading
+import uuid
 from pathlib import Path
 from typing import Callable, List, Literal
 
@@ -69,7 +71,9 @@ def create_app(msa_fn: Callable | None = None, root: str | os.PathLike = ".",
     if msa_fn is None:
         from .msa import msa as msa_fn
     root = Path(root)
-    chunks = int(os.environ.get("CHUNKS", 4)) if chunks is None else int(chunks)
+    if chunks is None:
+        from .db import chunks_tag
+        chunks = chunks_tag()
     extra = dict(msa_kwargs or {})
     lock = threading.Lock()
     result_store: dict[str, Path] = {}
@@ -80,8 +84,26 @@ def create_app(msa_fn: Callable | None = None, root: str | os.PathLike = ".",
     app.state.result_store = result_store
 
     def search(queries: list[str], filenames: list[str], **opts) -> None:
+        """Search the queries whose a3m is not yet in the cache.  The check
+        is repeated under the lock, so a sequence that two requests both
+        found missing is searched once; each a3m is written to a file of its
+        own and renamed into place, so a reader never sees a partial file
+        and a search never deletes a cached file another request reads."""
         with lock:
-            msa_fn(queries, filenames, **opts, **extra)
+            pending = dict(zip(filenames, queries))  # one search per file
+            todo = [(q, f) for f, q in pending.items() if not os.path.exists(f)]
+            if not todo:
+                return
+            tag = uuid.uuid4().hex
+            tmps = [f"{f}.{tag}.tmp" for _, f in todo]
+            try:
+                msa_fn([q for q, _ in todo], tmps, **opts, **extra)
+                for t, (_, f) in zip(tmps, todo):
+                    os.replace(t, f)
+            finally:
+                for t in tmps:
+                    if os.path.exists(t):
+                        os.remove(t)
 
     async def form_q(request: Request) -> str:
         """The form field q.  URL-encoded forms (what ColabFold/boltz clients
@@ -134,13 +156,15 @@ def create_app(msa_fn: Callable | None = None, root: str | os.PathLike = ".",
         tar_dir.mkdir(parents=True, exist_ok=True)
         tar_file = tar_dir / f"{tar_id}.tar.gz"
         name = "pair.a3m" if request.url.path.endswith("/ticket/pair") else "uniref.a3m"
-        with tarfile.open(tar_file, "w:gz") as tar:
+        tmp = tar_dir / f"{tar_id}.{uuid.uuid4().hex}.tmp"
+        with tarfile.open(tmp, "w:gz") as tar:
             ti = tarfile.TarInfo(name=name)
             ti.size = len(content)
             tar.addfile(ti, io.BytesIO(content))
             ti2 = tarfile.TarInfo(name=ENV_A3M)
             ti2.size = 0
             tar.addfile(ti2, io.BytesIO(b""))
+        os.replace(tmp, tar_file)
         result_store[tar_id] = tar_file
         return {"status": "COMPLETE", "id": tar_id}
 

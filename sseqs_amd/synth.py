@@ -223,3 +223,57 @@ def shuffled_copy(db, seed: int = 3):
         del seg, pos
     del asc
     return SeqDB.from_packed(ops.compress(out), total), perm
+
+
+# ------------------------------------------------------- benchmark set-up --
+
+# The benchmark database holds planted homologs of three query sets, one
+# homolog slot per block of BENCH_HOM_PERIOD sequences (12.5 %), the slot's
+# query drawn uniformly from a list in which the 200- and 500-residue
+# queries appear BENCH_REPEAT times each and every batched query once:
+# 512 / 1280 of 12.5 % = 5 % of the database for each single query (the
+# share the single-query workloads state), and 12.5 % / 1280 (about
+# 19 500 sequences of 200 M) for each of the 256 batched queries.
+BENCH_HOM_PERIOD = 8
+BENCH_REPEAT = 512
+BENCH_BATCH = 256
+
+
+def batch_query_lengths(n: int = BENCH_BATCH, seed: int = 1, lo: int = 50,
+                        hi: int = 1000) -> np.ndarray:
+    """Lengths of the batched workload's queries, uniform in [lo, hi]."""
+    rng = np.random.default_rng(seed)
+    return rng.integers(lo, hi + 1, size=n)
+
+
+def bench_queries(seed: int = 1, n_batch: int = BENCH_BATCH) -> tuple[str, str, list[str]]:
+    """(200-residue query, 500-residue query, batched queries)."""
+    q200 = make_query(200, 0, seed=seed)
+    q500 = make_query(500, 1, seed=seed)
+    lens = batch_query_lengths(n_batch, seed)
+    batch = [make_query(int(L), 2 + i, seed=seed) for i, L in enumerate(lens)]
+    return q200, q500, batch
+
+
+def bench_query_list(q200: str, q500: str, batch: list[str]) -> tuple[list[str], np.ndarray]:
+    """The generator's query list and, per list entry, the workload query it
+    stands for (0: q200, 1: q500, 2 + i: batched query i)."""
+    qs = [q200] * BENCH_REPEAT + [q500] * BENCH_REPEAT + list(batch)
+    owner = np.concatenate([np.zeros(BENCH_REPEAT, np.int64), np.ones(BENCH_REPEAT, np.int64),
+                            2 + np.arange(len(batch), dtype=np.int64)])
+    return qs, owner
+
+
+def bench_db(n: int, device="cuda", seed: int = 0, query_seed: int = 1):
+    """The benchmark database of n sequences and its queries: returns
+    (SeqDB, q200, q500, batch).  db.extra["hom_owner"] holds, per sequence,
+    the workload query it is a homolog of (-1 for background)."""
+    from .db import SeqDB
+    q200, q500, batch = bench_queries(query_seed)
+    qs, owner = bench_query_list(q200, q500, batch)
+    db = SeqDB.synthetic(n, qs, seed=seed, device=device, hom_period=BENCH_HOM_PERIOD)
+    hq = db.extra["hom_query"]
+    own = torch.from_numpy(owner.astype(np.int32)).to(hq.device)
+    db.extra["hom_owner"] = torch.where(hq >= 0, own[hq.clamp_min(0).long()], -1)
+    db.extra["gen_queries"] = qs
+    return db, q200, q500, batch

@@ -30,9 +30,7 @@ def big(dev):
     from sseqs_amd import synth
     from sseqs_amd.db import SeqDB
     announce("generate 200M-sequence database")
-    q200 = synth.make_query(200, 0, seed=1)
-    q500 = synth.make_query(500, 1, seed=1)
-    db = SeqDB.synthetic(N_FULL, [q200, q500], seed=0, device=dev)
+    db, q200, q500, _ = synth.bench_db(N_FULL, device=dev)
     torch.cuda.synchronize()
     yield dict(db=db, queries=[q200, q500])
     del db
@@ -55,7 +53,8 @@ def test_full_index_and_content(orc, big):
                                     np.arange(db.n - 50, db.n), past[:20],
                                     rng.choice(past, 300, replace=False)]))
     cls = db.extra["classes"]
-    want = orc.gen_seqs(synth.gen_params(0), big["queries"], cls.start, cls.length, ids)
+    want = orc.gen_seqs(synth.gen_params(0, hom_period=synth.BENCH_HOM_PERIOD),
+                        db.extra["gen_queries"], cls.start, cls.length, ids)
     got = db.strings(torch.from_numpy(ids).to(db.device), 1024)
     assert got == want
 
@@ -68,8 +67,12 @@ def test_full_cascade_q500(orc, big):
     q = prepare(query)
     gh = global_search(db, q, top_prefilter=100_000, top_hits=4096)
     ids = gh.ids.cpu().numpy()
-    hq = db.extra["hom_query"].cpu().numpy()[ids]
+    hq = db.extra["hom_owner"].cpu().numpy()[ids]
     assert np.mean(hq[:1000] == 1) > 0.95
+    # each single query owns 5 % of the database
+    own = db.extra["hom_owner"]
+    assert abs(float((own == 0).float().mean()) - 0.05) < 1e-3
+    assert abs(float((own == 1).float().mean()) - 0.05) < 1e-3
     # SW of the hits against the oracle
     sel = ids[:64]
     rows = db.strings(torch.from_numpy(sel).to(db.device), 1024)
@@ -81,11 +84,23 @@ def test_full_cascade_q500(orc, big):
     rows = db.strings(torch.from_numpy(samp).to(db.device), 1024)
     A, B = orc.weighted_luts(query, 4)
     qrev = orc.encode_query_rev(query)
-    want = np.array([orc.corr(qrev, A, B, 4, r.encode(), [0], [len(r)], 10**6,
-                              orc.nextpow2(max(len(query), len(r), 16)))[0] for r in rows])
+    args = [(r.encode(), [0], [len(r)], 10**6, orc.nextpow2(max(len(query), len(r), 16)))
+            for r in rows]
+    want = np.array([orc.corr(qrev, A, B, 4, *a)[0] for a in args])
+    tol = np.array([orc.corr_tol(qrev, A, B, 4, *a)[0] for a in args])
     got = gh.prefilter.cpu().numpy()[samp]
-    scale = 1e-3 * np.max(np.abs(want))
-    assert np.all(np.abs(got - want) <= 1e-2 * np.abs(want) + scale)
+    assert np.all(np.abs(got - want) <= tol)
+    # the same cascade with linear-gap SW (-4): the same candidates, the
+    # hits' linear scores bit-exact, and still the query's homologs
+    announce("global cascade q_len=500 linear -4 over 200M sequences")
+    gl = global_search(db, q, top_prefilter=100_000, top_hits=4096, linear_gap=-4)
+    assert torch.equal(gl.candidates, gh.candidates)
+    lids = gl.ids.cpu().numpy()
+    rows = db.strings(torch.from_numpy(lids[:64]).to(db.device), 1024)
+    assert gl.scores[:64].cpu().tolist() == [orc.sw_linear_one(qc, r, -4) for r in rows]
+    sc = gl.scores.cpu().numpy()
+    assert np.all(sc[:-1] >= sc[1:])
+    assert np.mean(db.extra["hom_owner"].cpu().numpy()[lids[:1000]] == 1) > 0.95
 
 
 def test_full_batch_cascade_q200(orc, big):
@@ -120,8 +135,9 @@ def test_full_batch_cascade_q200(orc, big):
         n = min(max(orc.nextpow2(max(len(query), int(plan.maxlen[b]))), 16), 2048)
         want = np.array([orc.corr(qrev, A, B, 4, r.encode(), [0], [len(r)], 10**6, n)[0]
                          for r in rows])
-        scale = 1e-3 * np.max(np.abs(want))
-        assert np.all(np.abs(got[loc] - want) <= 1e-2 * np.abs(want) + scale), b
+        tol = np.array([orc.corr_tol(qrev, A, B, 4, r.encode(), [0], [len(r)], 10**6, n)[0]
+                        for r in rows])
+        assert np.all(np.abs(got[loc] - want) <= tol), b
     # hits: scores recomputed from their text where the text is whole
     qc = orc.encode_query_sw(query)
     checked = 0

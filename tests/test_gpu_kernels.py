@@ -2,9 +2,9 @@
 
 Integer work (codec, index, SW, top-k, traceback, generator) must be
 bit-exact.  The prefilter is floating point: the tolerance, against the
-FP64 closed form, is |x - x64| <= 1e-2 * |x64| + 1e-3 * max|x64| (the
-second term covers scores near zero; fp16 outputs add at most 2^-11
-relative).
+FP64 closed form, is |x - x64| <= 1e-2 * |x64| + the error bound of fp16
+operands with fp32 accumulation (orc.corr_tol; the second term matters for
+scores near zero; fp16 outputs add at most 2^-11 relative).
 """
 import numpy as np
 import pytest
@@ -35,9 +35,10 @@ def mutate(rng, s, sub=0.3, indel=0.03):
     return "".join(out) or "A"
 
 
-def corr_tol(got, want):
-    scale = 1e-3 * max(1.0, float(np.max(np.abs(want))))
-    return np.abs(got - want) <= 1e-2 * np.abs(want) + scale
+def corr_ok(got, want, tol):
+    """|x - x64| <= tol elementwise, tol from orc.corr_tol: 1e-2 relative plus
+    the error bound of fp16 operands with fp32 accumulation."""
+    return np.abs(got - want) <= tol
 
 
 # ----------------------------------------------------------------- codec --
@@ -314,12 +315,13 @@ def test_warp_corr_matches_fp64(orc, dev, N, k):
     off = np.concatenate([[0], np.cumsum(lens + 1)[:-1]]).astype(np.int32)
     maxlen = int(lens.max()) if k != 8 else int(lens.max()) // 2  # exercise the clamp
     want = orc.corr(qrev, A, B, k, flat, off, lens, maxlen, N)
+    tol = orc.corr_tol(qrev, A, B, k, flat, off, lens, maxlen, N, want)
     t = lambda x: torch.from_numpy(np.ascontiguousarray(x)).to(dev)
     scores = torch.zeros(len(segs), dtype=torch.float16, device=dev)
     ops.warp_corr(scores, t(qrev), t(A), t(B), t(np.frombuffer(flat, np.uint8).copy()), t(off),
                   t(lens), k, Lq, maxlen, N, torch.zeros(1, dtype=torch.float16, device=dev))
     got = scores.float().cpu().numpy()
-    ok = corr_tol(got, want)
+    ok = corr_ok(got, want, tol)
     assert ok.all(), (N, k, np.max(np.abs(got - want)), got[~ok][:5], want[~ok][:5])
 
 
@@ -341,10 +343,11 @@ def test_db_prefilter_sequences_longer_than_2048(orc, dev):
     order = db.orig_id.cpu().numpy() if db.orig_id is not None else np.arange(db.n)
     A, B = orc.weighted_luts(query, 4)
     qrev = orc.encode_query_rev(query)
-    want = np.array([orc.corr(qrev, A, B, 4, s.encode(), [0], [len(s)], 2048,
-                              min(2048, orc.nextpow2(max(300, len(s), 16))))[0] for s in seqs])
-    want = want[order]
-    assert corr_tol(got, want).all(), np.max(np.abs(got - want))
+    args = [(s.encode(), [0], [len(s)], 2048, min(2048, orc.nextpow2(max(300, len(s), 16))))
+            for s in seqs]
+    want = np.array([orc.corr(qrev, A, B, 4, *a)[0] for a in args])[order]
+    tol = np.array([orc.corr_tol(qrev, A, B, 4, *a)[0] for a in args])[order]
+    assert corr_ok(got, want, tol).all(), np.max(np.abs(got - want))
 
 
 @pytest.mark.parametrize("Lq,k", [(50, 4), (200, 4), (500, 4), (200, 1), (90, 3)])
@@ -386,11 +389,13 @@ def test_db_prefilter_any_index_order(orc, dev, Lq, k):
     A, B = orc.weighted_luts(query, k)
     qrev = orc.encode_query_rev(query)
     want = np.zeros(len(seqs))
+    tol = np.zeros(len(seqs))
     for i, s in enumerate(seqs):
         N = min(2048, orc.nextpow2(max(Lq, len(s), 16)))
         want[i] = orc.corr(qrev, A, B, k, s.encode(), [0], [len(s)], N, N)[0]
-    want = want[order]
-    assert corr_tol(got, want).all(), np.max(np.abs(got - want))
+        tol[i] = orc.corr_tol(qrev, A, B, k, s.encode(), [0], [len(s)], N, N, want[i:i + 1])[0]
+    want, tol = want[order], tol[order]
+    assert corr_ok(got, want, tol).all(), np.max(np.abs(got - want))
     # the same sequences already sorted: contiguous tiles (staged path)
     srt = [seqs[i] for i in order]
     text2 = ("@".join(srt) + "@").encode()

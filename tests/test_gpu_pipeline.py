@@ -31,16 +31,35 @@ def world(orc, dev):
 
 
 def _fp64_prefilter(orc, ascii, off, ln, query):
+    """FP64 prefilter scores of every sequence and their tolerances."""
     A, B = orc.weighted_luts(query, 4)
     qrev = orc.encode_query_rev(query)
     out = np.zeros(ln.size)
+    tol = np.zeros(ln.size)
     Lq = len(query)
     Ns = np.array([orc.nextpow2(max(Lq, int(L), 16)) for L in ln])
     for N in np.unique(Ns):
         sel = np.nonzero(Ns == N)[0]
-        out[sel] = orc.corr(qrev, A, B, 4, ascii, off[sel].astype(np.int32), ln[sel], int(N),
-                            int(N))
-    return out
+        args = (ascii, off[sel].astype(np.int32), ln[sel], int(N), int(N))
+        out[sel] = orc.corr(qrev, A, B, 4, *args)
+        tol[sel] = orc.corr_tol(qrev, A, B, 4, *args, out[sel])
+    return out, tol
+
+
+def check_selection(orc, dev_scores, want, tol, k):
+    """The device's top k (by its own scores) against the FP64 top k: the
+    same set except for sequences whose FP64 score is within tolerance of
+    the k-th FP64 score (an element of one set and not the other satisfies
+    x64(e) + tol(e) >= x64(f) - tol(f) for some f on the other side)."""
+    cand = np.sort(orc.topk_desc(np.asarray(dev_scores, np.float64), k))
+    ref = np.sort(orc.topk_desc(want, k))
+    if k == 0:
+        return cand
+    thr = np.sort(want)[::-1][k - 1]
+    diff = np.setxor1d(cand, ref)
+    if diff.size:
+        assert np.all(np.abs(want[diff] - thr) <= tol[diff] + tol[diff].max()), diff
+    return cand
 
 
 @pytest.mark.parametrize("qi", [0, 1])
@@ -52,17 +71,12 @@ def test_global_cascade(orc, world, qi):
     q = prepare(query)
     gh = global_search(db, q, top_prefilter=3000, top_hits=500)
     pre = gh.prefilter.cpu().numpy()
-    want = _fp64_prefilter(orc, world["ascii"], world["off"], world["ln"], query)
-    scale = 1e-3 * np.max(np.abs(want))
-    assert np.all(np.abs(pre - want) <= 1e-2 * np.abs(want) + scale)
+    want, tol = _fp64_prefilter(orc, world["ascii"], world["off"], world["ln"], query)
+    assert np.all(np.abs(pre - want) <= tol)
     # candidates: exact top-k of the device scores, and the FP64 top-k up
     # to threshold ties within the tolerance
     cand = gh.candidates.cpu().numpy()
-    assert np.array_equal(cand, np.sort(orc.topk_desc(pre.astype(np.float64), 3000)))
-    ref = np.sort(orc.topk_desc(want, 3000))
-    thr = np.sort(want)[::-1][2999]
-    diff = np.setxor1d(cand, ref)
-    assert np.all(np.abs(want[diff] - thr) <= 1e-2 * abs(thr) + scale)
+    assert np.array_equal(cand, check_selection(orc, pre, want, tol, 3000))
     # affine SW of every candidate: bit exact
     qc = orc.encode_query_sw(query)
     sw = orc.sw_affine_segments(qc, world["ascii"], world["off"][cand], world["ln"][cand])
@@ -145,14 +159,18 @@ def _check_batch_cascade(orc, db, ascii, query, bs, resident, **kw):
     got = batch_search(db, query, q, bs=bs, resident=resident,
                        on_scores=lambda j, s: dev_scores.__setitem__(j, s.float().cpu().numpy()),
                        **kw)
-    want, fft64 = orc.batch_cascade(ascii, query, bs * 8 // 5,
-                                    fft_scores=lambda j: dev_scores[j], **kw)
+    want, fft64, tol64 = orc.batch_cascade(ascii, query, bs * 8 // 5,
+                                           fft_scores=lambda j: dev_scores[j], with_tol=True,
+                                           **kw)
     nonempty = [j for j, w in enumerate(fft64) if w.size]
     assert sorted(dev_scores) == nonempty
+    top_fft = kw.get("top_fft", 200)
     for j in nonempty:
-        w = fft64[j]
-        scale = 1e-3 * np.max(np.abs(w))
-        assert np.all(np.abs(dev_scores[j] - w) <= 1e-2 * np.abs(w) + scale), j
+        w, tol = fft64[j], tol64[j]
+        assert np.all(np.abs(dev_scores[j] - w) <= tol), j
+        # each batch's stage-1 passed set (top 1/top_fft of the device's fp16
+        # scores) is the FP64-selected set up to threshold ties
+        check_selection(orc, dev_scores[j], w, tol, w.size // top_fft)
     assert len(got) == len(want) > 0
     for g, w in zip(got, want):
         assert (np.float32(g.norm), g.affine, g.linear, g.seq) == w
@@ -281,9 +299,16 @@ def test_msa_batch_a3m(orc, world, tmp_path):
     t = sseqs_amd.msa([query], [str(out)], num_msas=64, verbose=False, db=db, bs=500_000,
                       top_fft=20, top_sw=4, return_timings=True)
     assert isinstance(t, dict)
-    hits = rank_hits(batch_search(db, query, prepare(query), bs=500_000, top_fft=20, top_sw=4),
-                     64)
-    want = orc.a3m(query, [orc.backtrack(query, h.seq) for h in hits])
+    # the expected side is the oracle's cascade and the oracle's merge
+    # (orc.rank_hits), fed the device's fp16 prefilter scores
+    dev_scores = {}
+    got = batch_search(db, query, prepare(query), bs=500_000, top_fft=20, top_sw=4,
+                       on_scores=lambda j, s: dev_scores.__setitem__(j, s.float().cpu().numpy()))
+    want_hits, _ = orc.batch_cascade(world["ascii"], query, 500_000 * 8 // 5, top_fft=20,
+                                     top_sw=4, fft_scores=lambda j: dev_scores[j])
+    ranked = orc.rank_hits(want_hits, 64)
+    assert [h.seq for h in rank_hits(got, 64)] == [h[3] for h in ranked]
+    want = orc.a3m(query, [orc.backtrack(query, h[3]) for h in ranked])
     assert out.read_text() == want
 
 
@@ -342,3 +367,32 @@ def test_msa_long_query_whole_hits(orc, world, tmp_path):
     assert [len(x) for x in seqs] == db.seq_len[gh.ids].cpu().tolist()
     want = orc.a3m(query, [orc.backtrack(query, s) for s in seqs])
     assert out.read_text() == want
+
+
+def test_batch_cascade_large_stage2(orc, dev):
+    """Short sequences and top_fft = top_sw = 1: the stage-2 selects keep
+    about 30 000 and 14 000 targets per batch, past the sorted limits of the
+    segmented (4096) and single (16384) selects; both paths against the
+    oracle."""
+    from sseqs_amd.db import SeqDB
+    announce("batch cascade with stage-2 selections above 16384")
+    rng = np.random.default_rng(13)
+    aa = np.frombuffer(b"ARNDCQEGHILKMFPSTWYV", dtype=np.uint8)
+    query = rng.choice(aa, 30).tobytes().decode()
+    parts = []
+    for i in range(45_000):
+        if i % 30 == 0:
+            s = np.frombuffer(mutate(rng, query, sub=0.3).encode(), dtype=np.uint8)
+        else:
+            s = rng.choice(aa, int(rng.integers(8, 41)))
+        parts.append(s.tobytes())
+        parts.append(b"@")
+    a = np.frombuffer(b"".join(parts), dtype=np.uint8).copy()
+    db = SeqDB.from_packed(torch.from_numpy(orc.xbit_pack(a)).to(dev), a.size)
+    bs = 500_000
+    plan = db.batch_plan(bs)
+    assert plan.n_batches == 2
+    assert plan.nseg[0] - 2 > 16384 and 4096 < plan.nseg[1] - 2 <= 16384
+    kw = dict(top_fft=1, top_sw=1, top_sw_affine=2)
+    _check_batch_cascade(orc, db, a, query, bs, True, **kw)
+    _check_batch_cascade(orc, db, a, query, bs, False, **kw)

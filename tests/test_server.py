@@ -59,7 +59,9 @@ def test_ticket_msa_and_download(stub_app):
     assert queries == ["MKTAYIAK", "GGSLLV"]
     assert opts == dict(fft_rank=4, top_fft=200, top_sw=10, top_sw_affine=2, num_msas=4096,
                         bs=100_000_000)
-    assert files[0].endswith(f"cache_msa/4_4096_200_4_10_2/{sha('MKTAYIAK')}.a3m")
+    # written under a temporary name in the cache folder, then renamed
+    assert f"cache_msa/4_4096_200_4_10_2/{sha('MKTAYIAK')}.a3m." in files[0]
+    assert (root / "cache_msa/4_4096_200_4_10_2" / f"{sha('MKTAYIAK')}.a3m").exists()
 
     d = client.get(f"/result/download/{body['id']}")
     assert d.status_code == 200 and d.headers["content-type"] == "application/gzip"
@@ -142,3 +144,44 @@ def test_ticket_status_poll(stub_app):
     tid = r.json()["id"]
     assert client.get(f"/ticket/{tid}").json() == {"status": "COMPLETE", "id": tid}
     assert client.get("/ticket/nope").json() == {"status": "UNKNOWN", "id": "nope"}
+
+
+def test_concurrent_tickets_search_once(tmp_path, monkeypatch):
+    """Two requests for the same uncached sequence at once: one search, both
+    answered from the same complete file (no request sees a missing or
+    partial a3m)."""
+    import threading
+    import time
+
+    for k in ("MSAS_NUM", "MSAS_FFT_RANK", "MSAS_TOP_FFT", "MSAS_TOP_SW", "MSAS_TOP_SW_AFFINE"):
+        monkeypatch.delenv(k, raising=False)
+    calls = []
+
+    def slow(queries, filenames, **opts):
+        calls.append(list(queries))
+        for q, fn in zip(queries, filenames):
+            with open(fn, "w") as fh:
+                fh.write(f">original_query\n{q}\n")
+                fh.flush()
+                time.sleep(0.3)  # a reader of a half-written file would see this line only
+                fh.write(f">hit\n{q}\n")
+
+    app = create_app(msa_fn=slow, root=tmp_path, chunks=4)
+    out = [None, None]
+
+    def post(i):
+        with TestClient(app) as c:
+            r = c.post("/ticket/msa", data={"q": f">r{i}\nMKTAYIAK"})
+            out[i] = untar(c.get(f"/result/download/{r.json()['id']}").content)
+
+    ts = [threading.Thread(target=post, args=(i,)) for i in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert calls == [["MKTAYIAK"]]
+    # both tickets name the same tarball (its id is the sequence's hash, as
+    # in server.py), written whole by each request
+    for i in range(2):
+        text = out[i]["uniref.a3m"].decode()
+        assert text in (f">r{j}\nMKTAYIAK\n>hit\nMKTAYIAK\n" for j in range(2))
