@@ -201,6 +201,10 @@ def main() -> int:
     ap.add_argument("--shuffled", action="store_true",
                     help="time the q_len=200 prefilter and per-batch cascade on the database and on a "
                          "copy of its stream in random order (use with --n-seqs)")
+    ap.add_argument("--batched-queries", type=int, default=256,
+                    help="queries of the batched sweep (lengths uniform in [50, 1000]; 0 skips it)")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="run only the batched sweep measurement (for traces)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="run a plain device copy of BYTES bytes and exit (counter calibration)")
     args = ap.parse_args()
@@ -484,8 +488,78 @@ def main() -> int:
         log(json.dumps(res))
         return res
 
+    def measure_batched(queries: list[str]):
+        """The batched workload: every query's whole-database cascade
+        (prefilter, top --top-prefilter, affine SW 11/1, top --top-hits,
+        backtrack) in one sweep captured as a HIP graph; one replay timed
+        with events on the launch stream, and one eager pass with events
+        between its stages."""
+        from sseqs_amd.multi import MultiSearch
+        from sseqs_amd.search import sw_affine_ids
+        t0 = time.perf_counter()
+        ms = MultiSearch(db, queries, args.top_prefilter, args.top_hits)
+        ms.capture()  # one eager sweep, then the capture
+        torch.cuda.synchronize()
+        setup = time.perf_counter() - t0
+        log(f"batched sweep of {len(queries)} queries set up and captured in {setup:.1f} s")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        ms.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        graph_ms = e0.elapsed_time(e1)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
+        ev[0].record()
+        ms._prefilter_select()
+        ev[1].record()
+        ms._sw()
+        ev[2].record()
+        ms._hits()
+        ev[3].record()
+        ms._backtrack()
+        ev[4].record()
+        torch.cuda.synchronize()
+        st = [ev[i].elapsed_time(ev[i + 1]) for i in range(4)]
+        lens = [len(t) for t in queries]
+        cand_res = int(db.seq_len[ms.cand].to(torch.int64).sum().item())
+        cells = sum(int(ms.q_len[i]) * int(db.seq_len[ms.cand[i * ms.kc:(i + 1) * ms.kc]]
+                                              .to(torch.int64).sum().item())
+                    for i in range(ms.nq))
+        # scores bit-equal to single-query launches (sampled queries)
+        same = True
+        for i in range(0, ms.nq, max(1, ms.nq // 8)):
+            one = sw_affine_ids(db, ms.qs[i], ms.cand[i * ms.kc:(i + 1) * ms.kc])
+            same &= bool(torch.equal(one, ms.cs[i * ms.kc:(i + 1) * ms.kc]))
+        pf_flops = 0.0
+        for q in ms.qs:
+            for N, b, e in db.length_ranges(q.length):
+                H = N // 2
+                pf_flops += (e - b) * q.k * (10 * H * np.log2(H) + 20 * H)
+        sec = graph_ms / 1e3
+        res = {"queries": len(queries), "q_len_min": min(lens), "q_len_max": max(lens),
+               "q_len_mean": round(float(np.mean(lens)), 1),
+               "top_prefilter": ms.kc, "top_hits": ms.kh,
+               "sweep_ms_graph_replay": round(graph_ms, 1),
+               "ms_per_query": round(graph_ms / len(queries), 2),
+               "queries_per_s": round(len(queries) / sec, 2),
+               "search_gcups": round(sum(lens) * db.residues / sec / 1e9, 1),
+               "db_gb_per_s_per_query": round(db.nbytes * len(queries) / sec / 1e9, 1),
+               "stages_ms_eager": {"prefilter+select": round(st[0], 1), "sw": round(st[1], 2),
+                                   "select_hits": round(st[2], 3), "backtrack": round(st[3], 1)},
+               "prefilter_tflops_fft_equiv": round(pf_flops / (st[0] / 1e3) / 1e12, 1),
+               "sw_cells": cells, "sw_gcups": round(cells / (st[1] / 1e3) / 1e9, 1),
+               "sw_launches": len(ms.buckets) + len(ms.long_queries),
+               "sw_tuning": {k: v for k, v in ms.tuning.get("sw", {}).items()},
+               "sw_scores_equal_single_query_runs": same,
+               "candidate_residues": cand_res, "setup_s": round(setup, 1)}
+        log(json.dumps(res))
+        return res, ms
+
     if args.batch_only:
         measure_batch(q200)
+        return 0
+    if args.batched_only:
+        measure_batched(batch_queries[: args.batched_queries])
         return 0
     def side(name, fn):
         """An auxiliary measurement: a failure is logged and reported as null
@@ -508,6 +582,13 @@ def main() -> int:
         r500l, _ = measure(q500, linear_gap=-4)
         if not args.no_recall:
             recalls.append(side("recall q_len 500", lambda: recall(q500)))
+    rbatched = None
+    if args.batched_queries > 0:
+        got = side("batched sweep", lambda: measure_batched(batch_queries[: args.batched_queries]))
+        if got is not None:
+            rbatched = got[0]
+            del got
+            torch.cuda.empty_cache()
     cpu = None
     if not args.no_cpu:
         t0 = time.time()
@@ -567,6 +648,7 @@ def main() -> int:
         "stream_kernels": rates,
         "q500": r500,
         "q500_linear4": r500l,
+        "batched": rbatched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

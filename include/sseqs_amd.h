@@ -163,6 +163,27 @@ int sseqs_db_sw_linear(const uint8_t* packed, const int64_t* seq_off, const int3
                        const int64_t* cand, int64_t n, const uint8_t* query, int query_len,
                        int gap_val, int32_t* out, hipStream_t stream);
 
+/* Batched affine SW (reference stripe rule, gap_open/gap_extend in
+ * 0..16000) of many queries' candidate lists: query q is
+ * qcodes[q_off[q] .. q_off[q+1]) (codes 0..19, 1..1024 residues), its
+ * candidates cand[c_off[q] .. c_off[q+1]) (database ids), and the score of
+ * cand[c] goes to out[c] (-1 for an empty sequence), equal to what
+ * sseqs_db_sw_affine gives for that query.  One persistent launch over the
+ * work queue of one length bucket: item i is candidates
+ * [c_off[q] + item_j[i] * item, min(+ item, c_off[q+1])) of query
+ * q = item_q[i], and every item's query must have stripe width
+ * ceil(Lq / 32) == tpt (1..32).  grid blocks of 256 threads (0: 1024) take
+ * items in order through the device counter `counter` (one uint32, reset
+ * by the call's first kernel).  Device arrays: q_off, c_off (int64),
+ * item_q, item_j (int32).  The database-resident SW of the batched sweep;
+ * no single reference counterpart (the reference scores one query per
+ * launch, sseqs/sw.cu:372-425). */
+int sseqs_db_sw_batch(const uint8_t* packed, const int64_t* seq_off, const int32_t* seq_len,
+                      const uint8_t* qcodes, const int64_t* q_off, const int64_t* cand,
+                      const int64_t* c_off, const int32_t* item_q, const int32_t* item_j,
+                      int64_t n_items, int item, int tpt, int grid, int gap_open,
+                      int gap_extend, int32_t* out, uint32_t* counter, hipStream_t stream);
+
 /* ---------------------------------------------------------------- top-k -- */
 
 /* dtype codes for sseqs_topk */

@@ -248,17 +248,23 @@ def corr(qrev, lutA: np.ndarray, lutB: np.ndarray, k: int, flat, offsets, length
 def corr_tol(qrev, lutA: np.ndarray, lutB: np.ndarray, k: int, flat, offsets, lengths,
              maxlen: int, N: int, x64: np.ndarray | None = None) -> np.ndarray:
     """Per-sequence tolerance of a device prefilter score against the FP64
-    closed form: 1e-2 * |x64| plus the error that rounding the two LUT
-    operands to fp16 (relative 2^-11 each, so 2^-10 + 2^-22 per product) and
-    summing k * Lq products in fp32 (at most k * Lq * 2^-24 relative to the
-    absolute sum) can cause.  Both scale with the sum of |a_r| * |b_r| over
-    the terms of a shift, which is at most the closed form evaluated with
-    |LUT_A| and |LUT_B| (the maximum over shifts of those absolute sums)."""
+    closed form: 1e-2 * |x64| plus the error that rounding the two operands
+    a = 0.5 * LUT_A and b = LUT_B to fp16 and summing n = k * Lq products in
+    fp32 can cause.  An operand rounded to fp16 is off by at most 2^-11 of
+    itself plus 2^-25 (half the smallest subnormal, for values that
+    underflow), so a product is off by at most (2^-10 + 2^-22) |a||b| +
+    2^-25 (1 + 2^-11) (|a| + |b|) + 2^-50, and the fp32 sum adds at most
+    n * 2^-24 of the absolute sum.  The sum of |a_r| * |b_r| over the terms
+    of a shift is at most the closed form evaluated with |LUT_A| and
+    |LUT_B| (the maximum over shifts of those absolute sums)."""
     if x64 is None:
         x64 = corr(qrev, lutA, lutB, k, flat, offsets, lengths, maxlen, N)
     absum = corr(qrev, np.abs(lutA), np.abs(lutB), k, flat, offsets, lengths, maxlen, N)
     n_terms = int(k) * len(qrev)
-    return 1e-2 * np.abs(x64) + (2.0**-10 + 2.0**-22 + n_terms * 2.0**-24) * absum
+    amax = 0.5 * float(np.max(np.abs(np.asarray(lutA, np.float64)[:k])))
+    bmax = float(np.max(np.abs(np.asarray(lutB, np.float64))))
+    under = n_terms * (2.0**-25 * (1 + 2.0**-11) * (amax + bmax) + 2.0**-50)
+    return 1e-2 * np.abs(x64) + (2.0**-10 + 2.0**-22 + n_terms * 2.0**-24) * absum + under
 
 
 def corr_fft_emulation(qrev, lutA, lutB, k: int, seg_codes_fn, L: int, maxlen: int, N: int) -> float:

@@ -54,6 +54,8 @@ SIGNATURES = {
     "sseqs_db_prefilter_kernel": (_i32, [_i32, _i32, _i32, _i32, C.c_char_p, _i32]),
     "sseqs_db_sw_affine": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp, _i32, _i32, _i32, _vp, _vp]),
     "sseqs_db_sw_linear": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp, _i32, _i32, _vp, _vp]),
+    "sseqs_db_sw_batch": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32,
+                                 _i32, _i32, _i32, _vp, _vp, _vp]),
     "sseqs_topk_ws_bytes": (_sz, [_i64]),
     "sseqs_topk": (_i32, [_vp, _i32, _i64, _i64, _i32, _vp, _vp, _sz, _vp]),
     "sseqs_topk_segments": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32,

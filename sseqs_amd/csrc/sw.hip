@@ -211,25 +211,15 @@ __device__ inline int sw_dpp(int v) {
     return __builtin_amdgcn_update_dpp(v, v, CTRL, 0xf, 0xf, false);
 }
 
-template <int TPT, bool AFFINE, class Src>
-__global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
-                                                         const uint8_t* __restrict__ query,
-                                                         int Lq, int64_t n, int p1, int p2,
-                                                         int32_t* __restrict__ out, Blosum bl) {
+// Query profile of the packed kernel: s_prof[(c * NW + w) * 16 + ln] packs
+// the scores of residue c against columns (ln + 16h) * TPT + 4w .. +3 of
+// the query (h = 0 low halves, h = 1 high halves), -4 past the query end.
+template <int TPT>
+__device__ void sw16_profile(const uint8_t* __restrict__ query, int Lq,
+                             const int8_t* __restrict__ s_bl, uint4* __restrict__ s_prof,
+                             int tid, int nthr) {
     constexpr int NW = (TPT + 3) / 4;
-    __shared__ int8_t s_bl[400];
-    __shared__ uint4 s_prof[21 * NW * 16];
-    __shared__ uint8_t s_res[SW16_WAVES][4][SW_CH];
-    __shared__ uint8_t s_code[256];  // byte -> residue index (aa_index)
-
-    const int tid = threadIdx.x;
-    const int wid = tid >> 6, lane = tid & 63;
-    const int grp = lane >> 4, l = lane & 15;
-
-    for (int i = tid; i < 400; i += 64 * SW16_WAVES) s_bl[i] = bl.v[i];
-    for (int i = tid; i < 256; i += 64 * SW16_WAVES) s_code[i] = (uint8_t)aa_index((uint32_t)i);
-    __syncthreads();
-    for (int idx = tid; idx < 21 * NW * 16; idx += 64 * SW16_WAVES) {
+    for (int idx = tid; idx < 21 * NW * 16; idx += nthr) {
         const int c = idx / (NW * 16);
         const int rem = idx - c * NW * 16;
         const int w = rem >> 4, ln = rem & 15;
@@ -252,8 +242,19 @@ __global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
         }
         s_prof[idx] = make_uint4(v[0], v[1], v[2], v[3]);
     }
-    __syncthreads();
+}
 
+// Scores of the four targets of one quad (quad * 4 + grp, grp = lane / 16)
+// by one wave: out[g] for g < n.  `res` is this row group's residue buffer
+// of SW_CH bytes in LDS.
+template <int TPT, bool AFFINE, class Src>
+__device__ __forceinline__ void sw16_quad(const Src& src, int64_t quad, int64_t n,
+                                          int32_t* __restrict__ out,
+                                          const uint4* __restrict__ s_prof,
+                                          const uint8_t* __restrict__ s_code,
+                                          uint8_t* __restrict__ res, int lane, int p1, int p2) {
+    constexpr int NW = (TPT + 3) / 4;
+    const int grp = lane >> 4, l = lane & 15;
     const i16x2 zero = {0, 0};
     // byte selector of the diagonal operand: the rotated word as it is, or
     // for lane 0 (stripes 0 and 16) 0 below and the word's low half above
@@ -263,135 +264,233 @@ __global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
     // the gap costs as unsigned amounts: extend (affine), -gap (linear)
     const u16x2 u2 = {(unsigned short)p2, (unsigned short)p2};
     const u16x2 ug = {(unsigned short)-p1, (unsigned short)-p1};
+    const int64_t g = quad * 4 + grp;
+    int64_t start = 0;
+    int len = 0;
+    bool valid = false;
+    if (g < n) valid = src.target(g, start, len);
+    const int T = valid ? len : 0;
+    int Tmax = max(T, __shfl_xor(T, 16, 64));
+    Tmax = max(Tmax, __shfl_xor(Tmax, 32, 64));
+    Tmax = __builtin_amdgcn_readfirstlane(Tmax);  // uniform: loops run on scalars
+
+    // Mo holds M - open of the previous row: the one value both the next
+    // row (Y) and the next column (X) take from M
+    i16x2 Hp[TPT], Mo[TPT], Yp[TPT];
+#pragma unroll
+    for (int k = 0; k < TPT; ++k) {
+        Hp[k] = zero;
+        Mo[k] = zero - c1;
+        Yp[k] = zero;
+    }
+    i16x2 best = zero;
+
+    for (int c0 = 0; c0 < Tmax; c0 += SW_CH) {
+        const int rows = min(SW_CH, Tmax - c0);
+        // this wave's LDS reads of the previous rows are done before
+        // the rows are overwritten, and the writes before the reads
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // four rows per lane per step, loads first; a row past the
+        // target's end re-reads its last residue (or, for an empty or
+        // skipped target, the symbol at `start`, which is in bounds) and
+        // stores 20
+        for (int r0 = l; r0 < rows; r0 += 64) {
+            uint32_t sy[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                sy[u] = src.sym(start, max(min(c0 + r0 + 16 * u, T - 1), 0));
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int r = r0 + 16 * u;
+                if (r < rows) res[r] = c0 + r < T ? s_code[sy[u] & 255u] : (uint8_t)20;
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        // one row of the DP; several per step so that the register
+        // rotation of Hp between rows needs no copies
+        auto row = [&](const int r) {
+            const int c = res[r];
+            const uint4* pr = &s_prof[c * NW * 16 + l];
+            // row rotation by one lane (every lane of the row reads, so
+            // no old value is needed), then lane 0's halves rearranged
+            const uint32_t rot = (uint32_t)__builtin_amdgcn_mov_dpp(
+                (int)as_u32(Hp[TPT - 1]), 0x121, 0xf, 0xf, false);
+            i16x2 dg = as_i16x2(__builtin_amdgcn_perm(rot, rot, dg_sel));
+            if constexpr (AFFINE) {
+                i16x2 lMo = zero - c1, lX = zero;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) {
+                    const uint4 pw = pr[w * 16];
+                    const uint32_t pv[4] = {pw.x, pw.y, pw.z, pw.w};
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        const int k = 4 * w + b;
+                        if (k < TPT) {
+                            // m and x without their clamp at 0: m >= -4, and
+                            // x >= m_left - open, so neither drifts; only
+                            // max(m, x, y) with y >= 0 is used, which equals
+                            // the clamped recurrence's H
+                            const i16x2 sc = as_i16x2(pv[b]);
+                            const i16x2 m = dg + sc;
+                            const i16x2 x = vmax(lMo, lX - c2);
+                            // max(M_up - open, Yp - extend, 0): Yp >= 0, so
+                            // the saturated second term carries the 0
+                            const i16x2 y = vmax(Mo[k], subsat(Yp[k], u2));
+                            const i16x2 h = vmax(m, vmax(x, y));
+                            const i16x2 mo = m - c1;
+                            dg = Hp[k];
+                            Hp[k] = h;
+                            Mo[k] = mo;
+                            Yp[k] = y;
+                            lMo = mo;
+                            lX = x;
+                            best = vmax(best, h);
+                        }
+                    }
+                }
+            } else {
+                i16x2 left = zero;
+#pragma unroll
+                for (int w = 0; w < NW; ++w) {
+                    const uint4 pw = pr[w * 16];
+                    const uint32_t pv[4] = {pw.x, pw.y, pw.z, pw.w};
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        const int k = 4 * w + b;
+                        if (k < TPT) {
+                            const i16x2 sc = as_i16x2(pv[b]);
+                            // max(dg + sc, Hp + gap, left + gap, 0): Hp and
+                            // left are >= 0, so max(Hp, left) + gap
+                            // saturated at 0 carries the 0
+                            const i16x2 h = vmax(dg + sc, subsat(vmax(Hp[k], left), ug));
+                            dg = Hp[k];
+                            Hp[k] = h;
+                            left = h;
+                            best = vmax(best, h);
+                        }
+                    }
+                }
+            }
+        };
+        // short stripes: four rows per step (the per-row overhead is a
+        // larger share of a short row); long stripes keep their registers
+        constexpr int RS = TPT <= 8 ? 4 : 2;
+        int r = 0;
+        for (; r + RS - 1 < rows; r += RS) {
+#pragma unroll
+            for (int u = 0; u < RS; ++u) row(r + u);
+        }
+        for (; r < rows; ++r) row(r);
+    }
+    int bv = max((int)best.x, (int)best.y);
+    bv = max(bv, sw_dpp<0x128>(bv));
+    bv = max(bv, sw_dpp<0x124>(bv));
+    bv = max(bv, sw_dpp<0x122>(bv));
+    bv = max(bv, sw_dpp<0x121>(bv));
+    if (l == 0 && g < n) out[g] = valid ? bv : -1;
+}
+
+template <int TPT, bool AFFINE, class Src>
+__global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16(Src src,
+                                                         const uint8_t* __restrict__ query,
+                                                         int Lq, int64_t n, int p1, int p2,
+                                                         int32_t* __restrict__ out, Blosum bl) {
+    constexpr int NW = (TPT + 3) / 4;
+    __shared__ int8_t s_bl[400];
+    __shared__ uint4 s_prof[21 * NW * 16];
+    __shared__ uint8_t s_res[SW16_WAVES][4][SW_CH];
+    __shared__ uint8_t s_code[256];  // byte -> residue index (aa_index)
+
+    const int tid = threadIdx.x;
+    const int wid = tid >> 6, lane = tid & 63;
+    const int grp = lane >> 4;
+
+    for (int i = tid; i < 400; i += 64 * SW16_WAVES) s_bl[i] = bl.v[i];
+    for (int i = tid; i < 256; i += 64 * SW16_WAVES) s_code[i] = (uint8_t)aa_index((uint32_t)i);
+    __syncthreads();
+    sw16_profile<TPT>(query, Lq, s_bl, s_prof, tid, 64 * SW16_WAVES);
+    __syncthreads();
     uint8_t* res = s_res[wid][grp];
     for (int64_t quad = (int64_t)blockIdx.x * SW16_WAVES + wid; quad * 4 < n;
-         quad += (int64_t)gridDim.x * SW16_WAVES) {
-        const int64_t g = quad * 4 + grp;
-        int64_t start = 0;
-        int len = 0;
-        bool valid = false;
-        if (g < n) valid = src.target(g, start, len);
-        const int T = valid ? len : 0;
-        int Tmax = max(T, __shfl_xor(T, 16, 64));
-        Tmax = max(Tmax, __shfl_xor(Tmax, 32, 64));
-        Tmax = __builtin_amdgcn_readfirstlane(Tmax);  // uniform: loops run on scalars
+         quad += (int64_t)gridDim.x * SW16_WAVES)
+        sw16_quad<TPT, AFFINE>(src, quad, n, out, s_prof, s_code, res, lane, p1, p2);
+}
 
-        // Mo holds M - open of the previous row: the one value both the next
-        // row (Y) and the next column (X) take from M
-        i16x2 Hp[TPT], Mo[TPT], Yp[TPT];
-#pragma unroll
-        for (int k = 0; k < TPT; ++k) {
-            Hp[k] = zero;
-            Mo[k] = zero - c1;
-            Yp[k] = zero;
-        }
-        i16x2 best = zero;
+// ---------------------------------------------------------------------------
+// Batched sweep: affine SW of many queries' candidate lists, one persistent
+// launch per stripe width (length bucket) present.
+//
+// Work item i is chunk item_j[i] (of `item` candidates) of query item_q[i]'s
+// candidate list cand[c_off[q] .. c_off[q+1]); the score of candidate c
+// goes to out[c].  The host sorts the items of a class by stripe width
+// and query; each stripe width (ceil(Lq / 32)) has its own queue.  A block takes the next item from `counter` (one global
+// atomic by one lane per item) and builds the query profile in LDS only
+// when the item's query differs from the previous one; its waves then score
+// the item's candidates four at a time with the same code as k_sw16, so a
+// candidate's score is the one a single-query launch gives.  One kernel
+// instance per stripe width keeps each bucket at k_sw16's register budget
+// (one instance covering several widths is sized for the widest and its
+// occupancy drops to 1-2 waves per SIMD).  Every block leaves the loop when
+// the queue is empty; the counter is reset by k_sw_reset before the launch.
+struct SwBatch {
+    const uint8_t* packed;
+    const int64_t* off;
+    const int32_t* len;
+    const uint8_t* qcodes;  // query q: qcodes[q_off[q] .. q_off[q+1]), codes 0..19
+    const int64_t* q_off;
+    const int64_t* cand;  // candidates of query q: cand[c_off[q] .. c_off[q+1])
+    const int64_t* c_off;
+    const int32_t* item_q;
+    const int32_t* item_j;
+    int64_t n_items;
+    int item;
+    uint32_t* counter;
+    int p1, p2;
+    int32_t* out;
+};
 
-        for (int c0 = 0; c0 < Tmax; c0 += SW_CH) {
-            const int rows = min(SW_CH, Tmax - c0);
-            // this wave's LDS reads of the previous rows are done before
-            // the rows are overwritten, and the writes before the reads
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            // four rows per lane per step, loads first; a row past the
-            // target's end re-reads its last residue (or, for an empty or
-            // skipped target, the symbol at `start`, which is in bounds) and
-            // stores 20
-            for (int r0 = l; r0 < rows; r0 += 64) {
-                uint32_t sy[4];
-#pragma unroll
-                for (int u = 0; u < 4; ++u)
-                    sy[u] = src.sym(start, max(min(c0 + r0 + 16 * u, T - 1), 0));
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int r = r0 + 16 * u;
-                    if (r < rows) res[r] = c0 + r < T ? s_code[sy[u] & 255u] : (uint8_t)20;
-                }
-            }
-            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            // one row of the DP; several per step so that the register
-            // rotation of Hp between rows needs no copies
-            auto row = [&](const int r) {
-                const int c = res[r];
-                const uint4* pr = &s_prof[c * NW * 16 + l];
-                // row rotation by one lane (every lane of the row reads, so
-                // no old value is needed), then lane 0's halves rearranged
-                const uint32_t rot = (uint32_t)__builtin_amdgcn_mov_dpp(
-                    (int)as_u32(Hp[TPT - 1]), 0x121, 0xf, 0xf, false);
-                i16x2 dg = as_i16x2(__builtin_amdgcn_perm(rot, rot, dg_sel));
-                if constexpr (AFFINE) {
-                    i16x2 lMo = zero - c1, lX = zero;
-#pragma unroll
-                    for (int w = 0; w < NW; ++w) {
-                        const uint4 pw = pr[w * 16];
-                        const uint32_t pv[4] = {pw.x, pw.y, pw.z, pw.w};
-#pragma unroll
-                        for (int b = 0; b < 4; ++b) {
-                            const int k = 4 * w + b;
-                            if (k < TPT) {
-                                // m and x without their clamp at 0: m >= -4, and
-                                // x >= m_left - open, so neither drifts; only
-                                // max(m, x, y) with y >= 0 is used, which equals
-                                // the clamped recurrence's H
-                                const i16x2 sc = as_i16x2(pv[b]);
-                                const i16x2 m = dg + sc;
-                                const i16x2 x = vmax(lMo, lX - c2);
-                                // max(M_up - open, Yp - extend, 0): Yp >= 0, so
-                                // the saturated second term carries the 0
-                                const i16x2 y = vmax(Mo[k], subsat(Yp[k], u2));
-                                const i16x2 h = vmax(m, vmax(x, y));
-                                const i16x2 mo = m - c1;
-                                dg = Hp[k];
-                                Hp[k] = h;
-                                Mo[k] = mo;
-                                Yp[k] = y;
-                                lMo = mo;
-                                lX = x;
-                                best = vmax(best, h);
-                            }
-                        }
-                    }
-                } else {
-                    i16x2 left = zero;
-#pragma unroll
-                    for (int w = 0; w < NW; ++w) {
-                        const uint4 pw = pr[w * 16];
-                        const uint32_t pv[4] = {pw.x, pw.y, pw.z, pw.w};
-#pragma unroll
-                        for (int b = 0; b < 4; ++b) {
-                            const int k = 4 * w + b;
-                            if (k < TPT) {
-                                const i16x2 sc = as_i16x2(pv[b]);
-                                // max(dg + sc, Hp + gap, left + gap, 0): Hp and
-                                // left are >= 0, so max(Hp, left) + gap
-                                // saturated at 0 carries the 0
-                                const i16x2 h = vmax(dg + sc, subsat(vmax(Hp[k], left), ug));
-                                dg = Hp[k];
-                                Hp[k] = h;
-                                left = h;
-                                best = vmax(best, h);
-                            }
-                        }
-                    }
-                }
-            };
-            // short stripes: four rows per step (the per-row overhead is a
-            // larger share of a short row); long stripes keep their registers
-            constexpr int RS = TPT <= 8 ? 4 : 2;
-            int r = 0;
-            for (; r + RS - 1 < rows; r += RS) {
-#pragma unroll
-                for (int u = 0; u < RS; ++u) row(r + u);
-            }
-            for (; r < rows; ++r) row(r);
+__global__ void k_sw_reset(uint32_t* counter) { *counter = 0u; }
+
+template <int TPT, bool AFFINE>
+__global__ __launch_bounds__(64 * SW16_WAVES) void k_sw16_batch(SwBatch b, Blosum bl) {
+    constexpr int NW = (TPT + 3) / 4;
+    __shared__ int8_t s_bl[400];
+    __shared__ uint4 s_prof[21 * NW * 16];
+    __shared__ uint8_t s_res[SW16_WAVES][4][SW_CH];
+    __shared__ uint8_t s_code[256];
+    __shared__ int s_item;
+
+    const int tid = threadIdx.x;
+    const int wid = tid >> 6, lane = tid & 63;
+    const int grp = lane >> 4;
+    for (int i = tid; i < 400; i += 64 * SW16_WAVES) s_bl[i] = bl.v[i];
+    for (int i = tid; i < 256; i += 64 * SW16_WAVES) s_code[i] = (uint8_t)aa_index((uint32_t)i);
+    uint8_t* res = s_res[wid][grp];
+    int cur_q = -1;
+    for (;;) {
+        if (tid == 0) s_item = (int)min(atomicAdd(b.counter, 1u), 0x7fffffffu);
+        __syncthreads();  // also: every wave is done with the previous item
+        // uniform across the block: item fields are loaded as scalars
+        const int it = __builtin_amdgcn_readfirstlane(s_item);
+        if (it >= b.n_items) break;
+        const int q = __builtin_amdgcn_readfirstlane(b.item_q[it]);
+        const int64_t qb = b.q_off[q];
+        const int Lq = __builtin_amdgcn_readfirstlane((int)(b.q_off[q + 1] - qb));
+        if (q != cur_q) {
+            sw16_profile<TPT>(b.qcodes + qb, Lq, s_bl, s_prof, tid, 64 * SW16_WAVES);
+            cur_q = q;
         }
-        int bv = max((int)best.x, (int)best.y);
-        bv = max(bv, sw_dpp<0x128>(bv));
-        bv = max(bv, sw_dpp<0x124>(bv));
-        bv = max(bv, sw_dpp<0x122>(bv));
-        bv = max(bv, sw_dpp<0x121>(bv));
-        if (l == 0 && g < n) out[g] = valid ? bv : -1;
+        __syncthreads();
+        const int64_t cb = b.c_off[q] + (int64_t)__builtin_amdgcn_readfirstlane(b.item_j[it]) *
+                                            b.item;
+        const int64_t n = min(cb + (int64_t)b.item, b.c_off[q + 1]) - cb;
+        const SrcPacked src{b.packed, b.off, b.len, b.cand + cb};
+        for (int64_t quad = wid; quad * 4 < n; quad += SW16_WAVES)
+            sw16_quad<TPT, AFFINE>(src, quad, n, b.out + cb, s_prof, s_code, res, lane, b.p1,
+                                   b.p2);
+        __syncthreads();  // s_item is rewritten at the top
     }
 }
 
@@ -515,4 +614,44 @@ extern "C" int sseqs_db_sw_linear(const uint8_t* packed, const int64_t* seq_off,
     if (gap_val > 0) return fail(-1, "db_sw_linear: gap_val must be <= 0");
     SrcPacked src{packed, seq_off, seq_len, cand};
     return launch_sw<false>(src, query, Lq, n, gap_val, 0, out, stream);
+}
+
+extern "C" int sseqs_db_sw_batch(const uint8_t* packed, const int64_t* seq_off,
+                                 const int32_t* seq_len, const uint8_t* qcodes,
+                                 const int64_t* q_off, const int64_t* cand, const int64_t* c_off,
+                                 const int32_t* item_q, const int32_t* item_j, int64_t n_items,
+                                 int item, int tpt, int grid, int gap_open,
+                                 int gap_extend, int32_t* out, uint32_t* counter,
+                                 hipStream_t stream) {
+    if (n_items < 0 || item < 1 || grid < 0) return fail(-1, "db_sw_batch: bad sizes");
+    if (gap_open < 0 || gap_extend < 0 || gap_open > 16000 || gap_extend > 16000)
+        return fail(-1, "db_sw_batch: gap_open and gap_extend must be in 0..16000");
+    if (n_items == 0) return 0;
+    if (!packed || !seq_off || !seq_len || !qcodes || !q_off || !cand || !c_off || !item_q ||
+        !item_j || !out || !counter)
+        return fail(-1, "db_sw_batch: null pointer");
+    if (n_items > 0x7fffffffLL) return fail(-1, "db_sw_batch: too many work items");
+    const SwBatch b{packed, seq_off, seq_len, qcodes, q_off, cand, c_off, item_q, item_j,
+                    n_items, item, counter, gap_open, gap_extend, out};
+    const Blosum bl = blosum62();
+    const unsigned g = (unsigned)(grid > 0 ? grid : 1024);
+    hipLaunchKernelGGL(k_sw_reset, dim3(1), dim3(1), 0, stream, counter);
+    if (int r = check_launch("db_sw_batch reset")) return r;
+    switch (tpt) {
+#define SWB_CASE(T)                                                                            \
+    case T:                                                                                    \
+        hipLaunchKernelGGL((k_sw16_batch<T, true>), dim3(g), dim3(64 * SW16_WAVES), 0, stream, \
+                           b, bl);                                                             \
+        break;
+        SWB_CASE(1) SWB_CASE(2) SWB_CASE(3) SWB_CASE(4) SWB_CASE(5) SWB_CASE(6) SWB_CASE(7)
+        SWB_CASE(8) SWB_CASE(9) SWB_CASE(10) SWB_CASE(11) SWB_CASE(12) SWB_CASE(13)
+        SWB_CASE(14) SWB_CASE(15) SWB_CASE(16) SWB_CASE(17) SWB_CASE(18) SWB_CASE(19)
+        SWB_CASE(20) SWB_CASE(21) SWB_CASE(22) SWB_CASE(23) SWB_CASE(24) SWB_CASE(25)
+        SWB_CASE(26) SWB_CASE(27) SWB_CASE(28) SWB_CASE(29) SWB_CASE(30) SWB_CASE(31)
+        SWB_CASE(32)
+#undef SWB_CASE
+    default:
+        return fail(-1, "db_sw_batch: stripe width must be 1..32 (queries of 1..1024 residues)");
+    }
+    return check_launch("db_sw_batch");
 }

@@ -53,7 +53,7 @@ def _sha(s: str) -> str:
 
 
 def create_app(msa_fn: Callable | None = None, root: str | os.PathLike = ".",
-               chunks: int | None = None, msa_kwargs: dict | None = None):
+               chunks: int | None = None, msa_kwargs: dict | None = None, engine=None):
     """FastAPI app serving the reference routes.
 
     msa_fn: the search, called as ``msa_fn(queries, filenames, **options)``;
@@ -61,6 +61,14 @@ def create_app(msa_fn: Callable | None = None, root: str | os.PathLike = ".",
     and the tarballs. chunks: the database size tag of the cache folders
     (default ``$CHUNKS`` or 4, server.py:25). msa_kwargs: extra keyword
     arguments for every search (for example ``db=``).
+
+    engine: a :class:`sseqs_amd.serve.SearchEngine`.  Searches then go to
+    its continuous-batching device loop instead of ``msa_fn``: concurrent
+    requests share sweeps, no lock serialises them, and a sequence that
+    several requests wait for is searched once.  The engine's whole-database
+    cascade (its top_prefilter and top_hits) replaces the per-batch
+    cascade options of the request, and its cache folders are named
+    ``engine_<top_prefilter>_<top_hits>`` so the two never mix.
     """
     from fastapi import FastAPI, HTTPException, Request
     from fastapi.concurrency import run_in_threadpool
@@ -83,7 +91,39 @@ def create_app(msa_fn: Callable | None = None, root: str | os.PathLike = ".",
                        allow_methods=["*"], allow_headers=["*"])
     app.state.result_store = result_store
 
+    inflight: dict[str, object] = {}
+    inflight_lock = threading.Lock()
+
+    def write_atomic(path: str, text: str) -> None:
+        tmp = f"{path}.{uuid.uuid4().hex}.tmp"
+        with open(tmp, "w") as fh:
+            fh.write(text)
+        os.replace(tmp, path)
+
+    def engine_search(queries: list[str], filenames: list[str]) -> None:
+        waits = []
+        with inflight_lock:
+            for q, f in dict(zip(filenames, queries)).items():
+                if os.path.exists(f):
+                    continue
+                fut = inflight.get(f)
+                if fut is None:
+                    fut = engine.submit(q)
+                    inflight[f] = fut
+                waits.append((f, fut))
+        try:
+            for f, fut in waits:
+                write_atomic(f, fut.result())
+        finally:
+            with inflight_lock:
+                for f, fut in waits:
+                    if inflight.get(f) is fut and fut.done():
+                        del inflight[f]
+
     def search(queries: list[str], filenames: list[str], **opts) -> None:
+        if engine is not None:
+            engine_search(queries, filenames)
+            return
         """Search the queries whose a3m is not yet in the cache.  The check
         is repeated under the lock, so a sequence that two requests both
         found missing is searched once; each a3m is written to a file of its
@@ -131,7 +171,9 @@ def create_app(msa_fn: Callable | None = None, root: str | os.PathLike = ".",
         top_fft = int(os.getenv("MSAS_TOP_FFT", 200))
         top_sw = int(os.getenv("MSAS_TOP_SW", 10))
         top_sw_affine = int(os.getenv("MSAS_TOP_SW_AFFINE", 2))
-        folder = root / "cache_msa" / f"{chunks}_{num}_{top_fft}_{rank}_{top_sw}_{top_sw_affine}"
+        folder = root / "cache_msa" / (
+            f"{chunks}_{num}_{top_fft}_{rank}_{top_sw}_{top_sw_affine}" if engine is None else
+            f"{chunks}_engine_{engine.kc}_{engine.kh}")
         folder.mkdir(parents=True, exist_ok=True)
 
         heads, proteins = lines[::2], lines[1::2]
@@ -195,7 +237,8 @@ def create_app(msa_fn: Callable | None = None, root: str | os.PathLike = ".",
     @app.post("/msa")
     def get_msa(req: MSARequest):
         folder = root / (f"{chunks}_{req.msas_num}_{req.top_fft}_{req.fft_rank}_{req.top_sw}_"
-                         f"{req.top_sw_affine}")
+                         f"{req.top_sw_affine}" if engine is None else
+                         f"{chunks}_engine_{engine.kc}_{engine.kh}")
         folder.mkdir(parents=True, exist_ok=True)
         files = [str(folder / f"{_sha(p)}.a3m") for p in req.sequences]
 
