@@ -205,6 +205,18 @@ def main() -> int:
                     help="queries of the batched sweep (lengths uniform in [50, 1000]; 0 skips it)")
     ap.add_argument("--batched-only", action="store_true",
                     help="run only the batched sweep measurement (for traces)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned prefilter")
+    ap.add_argument("--learned-only", action="store_true",
+                    help="run only the learned prefilter measurement")
+    ap.add_argument("--recall-queries", type=int, default=4,
+                    help="batched queries whose Recall@4096 is measured against exhaustive SW")
+    ap.add_argument("--recall-timed", type=int, default=2,
+                    help="of those, how many get the plain cascade timed at its 0.99 point")
+    ap.add_argument("--serve-requests", type=int, default=400,
+                    help="requests of the serving measurement at the high rate (0 skips it)")
+    ap.add_argument("--serve-plain-requests", type=int, default=16)
+    ap.add_argument("--serve-rate-high", type=float, default=200.0)
+    ap.add_argument("--serve-rate-low", type=float, default=20.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="run a plain device copy of BYTES bytes and exit (counter calibration)")
     args = ap.parse_args()
@@ -488,7 +500,7 @@ def main() -> int:
         log(json.dumps(res))
         return res
 
-    def measure_batched(queries: list[str]):
+    def measure_batched(queries: list[str], proposer=None):
         """The batched workload: every query's whole-database cascade
         (prefilter, top --top-prefilter, affine SW 11/1, top --top-hits,
         backtrack) in one sweep captured as a HIP graph; one replay timed
@@ -497,17 +509,9 @@ def main() -> int:
         from sseqs_amd.multi import MultiSearch
         from sseqs_amd.search import sw_affine_ids
         t0 = time.perf_counter()
-        ms = MultiSearch(db, queries, args.top_prefilter, args.top_hits)
-        ms.capture()  # one eager sweep, then the capture
-        torch.cuda.synchronize()
-        setup = time.perf_counter() - t0
-        log(f"batched sweep of {len(queries)} queries set up and captured in {setup:.1f} s")
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        ms.replay()
-        e1.record()
-        torch.cuda.synchronize()
-        graph_ms = e0.elapsed_time(e1)
+        ms = MultiSearch(db, queries, args.top_prefilter, args.top_hits, proposer=proposer)
+        # the eager sweep (warm-up, with events between its stages), then
+        # the capture and one timed replay of the graph
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
         ev[0].record()
         ms._prefilter_select()
@@ -520,6 +524,19 @@ def main() -> int:
         ev[4].record()
         torch.cuda.synchronize()
         st = [ev[i].elapsed_time(ev[i + 1]) for i in range(4)]
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            ms.run()
+        ms.graph = g
+        torch.cuda.synchronize()
+        setup = time.perf_counter() - t0
+        log(f"batched sweep of {len(queries)} queries set up and captured in {setup:.1f} s")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        ms.replay()
+        e1.record()
+        torch.cuda.synchronize()
+        graph_ms = e0.elapsed_time(e1)
         lens = [len(t) for t in queries]
         cand_res = int(db.seq_len[ms.cand].to(torch.int64).sum().item())
         cells = sum(int(ms.q_len[i]) * int(db.seq_len[ms.cand[i * ms.kc:(i + 1) * ms.kc]]
@@ -535,6 +552,8 @@ def main() -> int:
             for N, b, e in db.length_ranges(q.length):
                 H = N // 2
                 pf_flops += (e - b) * q.k * (10 * H * np.log2(H) + 20 * H)
+        if proposer is not None:  # learned: the proposal GEMM's flops instead
+            pf_flops = 2.0 * 256 * len(queries) * db.n
         sec = graph_ms / 1e3
         res = {"queries": len(queries), "q_len_min": min(lens), "q_len_max": max(lens),
                "q_len_mean": round(float(np.mean(lens)), 1),
@@ -546,7 +565,8 @@ def main() -> int:
                "db_gb_per_s_per_query": round(db.nbytes * len(queries) / sec / 1e9, 1),
                "stages_ms_eager": {"prefilter+select": round(st[0], 1), "sw": round(st[1], 2),
                                    "select_hits": round(st[2], 3), "backtrack": round(st[3], 1)},
-               "prefilter_tflops_fft_equiv": round(pf_flops / (st[0] / 1e3) / 1e12, 1),
+               ("proposal_tflops" if proposer is not None else "prefilter_tflops_fft_equiv"):
+                   round(pf_flops / (st[0] / 1e3) / 1e12, 1),
                "sw_cells": cells, "sw_gcups": round(cells / (st[1] / 1e3) / 1e9, 1),
                "sw_launches": len(ms.buckets) + len(ms.long_queries),
                "sw_tuning": {k: v for k, v in ms.tuning.get("sw", {}).items()},
@@ -555,11 +575,164 @@ def main() -> int:
         log(json.dumps(res))
         return res, ms
 
+    def measure_learned(queries: list[str], plain_batched: dict | None):
+        """The learned prefilter on the batched workload: encoder weights
+        from sseqs_amd/encoder_weights.npz, the FP8 table of the whole
+        database (built once, timed), the batched sweep with the proposal
+        in place of the prefilter, and Recall@4096 against exhaustive SW of
+        the whole database for a sample of the queries, for both
+        prefilters, with each one's candidate count for 0.99 recall and the
+        time of the cascade run there."""
+        from sseqs_amd.learned import WEIGHTS_PATH, EmbeddingTable, Encoder, Proposer, exhaustive_top
+        from sseqs_amd.search import global_search
+        if not WEIGHTS_PATH.exists():
+            log("no trained encoder weights; learned prefilter skipped")
+            return None
+        enc = Encoder.load(WEIGHTS_PATH, device=dev)
+        tab = EmbeddingTable(enc, db, log=log)
+        build = {"seconds": round(tab.build_s, 1), "rows": tab.rows_total,
+                 "encoder_tflops": round(tab.flops / tab.build_s / 1e12, 1),
+                 "table_gb": round(tab.nbytes / 1e9, 1)}
+        log(f"embedding table {json.dumps(build)}")
+        prop = Proposer(tab)
+        res, ms = measure_batched(queries, proposer=prop)
+        learned_hits = ms.hit_sc.view(ms.nq, ms.kh)
+        kh = ms.kh
+        # Recall@4096 on a sample of the queries, both prefilters
+        k = min(4096, kh)
+        sample = np.linspace(0, len(queries) - 1, args.recall_queries).round().astype(int)
+        sample = np.unique(sample)
+        rec = []
+        grid = (100_000, 300_000, 1_000_000, 3_000_000, 10_000_000, 30_000_000, 100_000_000)
+        for i in sample:
+            qtext = queries[i]
+            t = time.perf_counter()
+            kth, allsc = exhaustive_top(db, qtext, k)
+            torch.cuda.synchronize()
+            t_ex = time.perf_counter() - t
+            good = allsc >= kth
+            r = {"q_len": len(qtext), "kth_exhaustive_score": kth,
+                 "exhaustive_sw_s": round(t_ex, 2),
+                 "learned_recall_at_top_prefilter": round(
+                     min(k, int((learned_hits[i] >= kth).sum().item())) / k, 4)}
+            q = prepare(qtext, device=dev)
+            gh = global_search(db, q, args.top_prefilter, k, scores_buf=scores_buf)
+            r["plain_recall_at_top_prefilter"] = round(
+                min(k, int((gh.scores >= kth).sum().item())) / k, 4)
+            need = -(-99 * k // 100)
+            for name, sc in (("plain", gh.prefilter), ("learned", prop.scores_for([qtext])[0])):
+                order = torch.sort(sc.float(), descending=True, stable=True).indices
+                cum = torch.cumsum(good[order].to(torch.int32), 0)
+                del order
+                r[f"{name}_recall_curve"] = {str(K): round(min(k, int(cum[K - 1].item())) / k, 4)
+                                             for K in grid if K <= db.n}
+                K99 = int(torch.searchsorted(cum, torch.tensor([need], dtype=cum.dtype,
+                                                               device=dev)).item()) + 1
+                del cum
+                r[f"{name}_top_prefilter_for_0.99"] = K99
+            # the plain cascade at its 0.99 operating point, timed (for the
+            # first --recall-timed queries of the sample)
+            if len(rec) < args.recall_timed:
+                K99 = r["plain_top_prefilter_for_0.99"]
+                global_search(db, q, K99, k, scores_buf=scores_buf)
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                g99 = global_search(db, q, K99, k, scores_buf=scores_buf)
+                torch.cuda.synchronize()
+                r["plain_ms_at_0.99"] = round(1e3 * (time.perf_counter() - t), 1)
+                r["plain_recall_measured_at_0.99"] = round(
+                    min(k, int((g99.scores >= kth).sum().item())) / k, 4)
+                del g99
+            del allsc, good, gh
+            log(json.dumps(r))
+            rec.append(r)
+        out = {"table": build, "sweep": res, "recall_sample": rec,
+               "learned_ms_per_query": res["ms_per_query"],
+               "plain_ms_per_query_at_0.99": round(float(np.mean([r["plain_ms_at_0.99"]
+                                                                  for r in rec
+                                                                  if "plain_ms_at_0.99" in r])), 1),
+               "recall_queries": len(rec),
+               "learned_recall_min": min(r["learned_recall_at_top_prefilter"] for r in rec),
+               "learned_recall_mean": round(float(np.mean(
+                   [r["learned_recall_at_top_prefilter"] for r in rec])), 4),
+               "plain_recall_mean": round(float(np.mean(
+                   [r["plain_recall_at_top_prefilter"] for r in rec])), 4),
+               "learned_k99_max": max(r["learned_top_prefilter_for_0.99"] for r in rec),
+               "plain_k99_median": int(np.median([r["plain_top_prefilter_for_0.99"]
+                                                  for r in rec]))}
+        if plain_batched:
+            out["plain_batched_ms_per_query_top_prefilter"] = plain_batched["ms_per_query"]
+        out["speedup_vs_plain_at_0.99"] = round(out["plain_ms_per_query_at_0.99"] /
+                                                out["learned_ms_per_query"], 1)
+        log(json.dumps({k_: v for k_, v in out.items() if k_ not in ("sweep", "recall_sample")}))
+        del ms
+        torch.cuda.empty_cache()
+        if args.serve_requests > 0:
+            # serving at each prefilter's 0.99-recall operating point
+            k_l = max(args.top_prefilter, out["learned_k99_max"])
+            k_p = max(r["plain_top_prefilter_for_0.99"] for r in rec)
+            out["serving"] = [
+                side("serving learned", lambda: measure_serving(
+                    args.serve_rate_high, args.serve_requests, k_l, prop, "learned")),
+                side("serving learned", lambda: measure_serving(
+                    args.serve_rate_low, max(16, args.serve_requests // 10), k_l, prop,
+                    "learned")),
+                side("serving plain", lambda: measure_serving(
+                    args.serve_rate_low, args.serve_plain_requests, k_p, None, "plain")),
+            ]
+        del tab, prop
+        torch.cuda.empty_cache()
+        return out
+
+    def measure_serving(rate: float, n_req: int, top_prefilter: int, proposer=None,
+                        label: str = ""):
+        """The continuous-batching engine under Poisson arrivals: n_req
+        requests at `rate` per second, query lengths log-normal (median 300,
+        sigma 0.5) clipped to [30, 1000], each answered with its a3m text;
+        latency is submit to result."""
+        from sseqs_amd.serve import SearchEngine
+        rng = np.random.default_rng(int(rate) * 7 + n_req)
+        lens = np.clip(np.round(np.exp(rng.normal(np.log(300), 0.5, n_req))), 30, 1000)
+        qs = [synth.make_query(int(L), 50_000 + i, seed=3) for i, L in enumerate(lens)]
+        gaps = rng.exponential(1.0 / rate, n_req)
+        with SearchEngine(db, top_prefilter=top_prefilter, top_hits=args.top_hits, max_batch=64,
+                          proposer=proposer) as eng:
+            eng.search(qs[0], timeout=600)  # warm-up
+            eng.latencies.clear()
+            eng.stats.batch_sizes.clear()
+            t0 = time.perf_counter()
+            futs = []
+            t_next = t0
+            for q, gap in zip(qs, gaps):
+                t_next += gap
+                delay = t_next - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                futs.append(eng.submit(q))
+            rows = [f.result(timeout=1200).count(">") for f in futs]
+            t1 = time.perf_counter()
+            lat = np.array(eng.latencies) * 1e3
+            bs = list(eng.stats.batch_sizes)
+        res = {"prefilter": label, "offered_req_per_s": rate, "requests": n_req,
+               "top_prefilter": top_prefilter,
+               "achieved_req_per_s": round(n_req / (t1 - t0), 2),
+               "p50_ms": round(float(np.percentile(lat, 50)), 1),
+               "p99_ms": round(float(np.percentile(lat, 99)), 1),
+               "mean_sweep_size": round(float(np.mean(bs)), 2), "sweeps": len(bs),
+               "mean_a3m_rows": round(float(np.mean(rows)), 1)}
+        log(json.dumps(res))
+        return res
+
     if args.batch_only:
         measure_batch(q200)
         return 0
+    if args.learned_only:
+        print(json.dumps({"learned": measure_learned(batch_queries[: args.batched_queries],
+                                                     None)}), flush=True)
+        return 0
     if args.batched_only:
-        measure_batched(batch_queries[: args.batched_queries])
+        print(json.dumps({"batched": measure_batched(batch_queries[: args.batched_queries])[0]}),
+              flush=True)
         return 0
     def side(name, fn):
         """An auxiliary measurement: a failure is logged and reported as null
@@ -589,6 +762,10 @@ def main() -> int:
             rbatched = got[0]
             del got
             torch.cuda.empty_cache()
+    rlearned = None
+    if args.batched_queries > 0 and not args.no_learned:
+        rlearned = side("learned prefilter",
+                        lambda: measure_learned(batch_queries[: args.batched_queries], rbatched))
     cpu = None
     if not args.no_cpu:
         t0 = time.time()
@@ -649,6 +826,7 @@ def main() -> int:
         "q500": r500,
         "q500_linear4": r500l,
         "batched": rbatched,
+        "learned": rlearned,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

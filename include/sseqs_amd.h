@@ -237,6 +237,76 @@ int sseqs_backtrack(const uint8_t* query, int query_len, const uint8_t* targets,
                     uint8_t* out_q, uint8_t* out_t, int32_t* info, void* ws, size_t ws_bytes,
                     hipStream_t stream);
 
+/* -------------------------------------------------- learned prefilter -- */
+
+/* The encoder of the learned prefilter and its candidate proposal.  None of
+ * these has a reference counterpart: the reference has only the FFT
+ * prefilter (sseqs/msa.py:270-310), whose stage-1 selection the proposal
+ * replaces.  Encoder: embedding 22 x 64 (codes 0..19 residues, 20 other,
+ * 21 pad; rows 20, 21 zero), three conv1d layers of width 9
+ * (64 -> 128 -> 128 -> 256) with GELU (tanh form), mean pool, L2 norm.
+ * A batch is a flat array of M rows (M a multiple of 128): four pad rows
+ * (mask 0) before every sequence and after the last.  Activation buffers
+ * are bf16 row-major [rows][C]; a pointer "to row 0" of a buffer that a
+ * conv reads as its window source (A, X) must have four readable zero
+ * rows before row 0 and after row M - 1.  Scores and gradients are fp32.
+ *
+ * sseqs_enc_conv: C[p][n] = sum_k A[(p - 4) * lda + k] * Bt[n * K + k] for
+ * p < M, n < N (N a multiple of 64, K of 32), then by `epi`:
+ *   0: U[p][n] = C + bias[n] (if U), Y[p][n] = gelu(U) * mask[p];
+ *   1: Y[p][n] = C * gelu'(Uprev[p][n]) * mask[p];
+ *   2: Yf[p][n] = C * mask[p] (fp32).
+ * Forward layers use Bt[o][t * Cin + i] = W[t][i][o]; the input gradient
+ * uses A = dU and Bt[i][t * Cout + o] = W[8 - t][i][o]
+ * (sseqs_enc_pack_weights writes both from W[t][i][o]). */
+int sseqs_enc_conv(const void* A, int lda, const void* Bt, int64_t M, int N, int K,
+                   const float* bias, const uint8_t* mask, void* U, void* Y, const void* Uprev,
+                   float* Yf, int epi, hipStream_t stream);
+/* dW[t * Cin + i][o] += sum_{p < M} X[(p - 4) * Cin + t * Cin + i] * dU[p][o]
+ * (Cin, Cout multiples of 64). */
+int sseqs_enc_wgrad(const void* X, int Cin, const void* dU, int Cout, int64_t M, float* dW,
+                    hipStream_t stream);
+/* db[c] += sum_{p < M} dU[p][c] (C a multiple of 64). */
+int sseqs_enc_bias_grad(const void* dU, int C, int64_t M, float* db, hipStream_t stream);
+/* x0[p] = E[code[p]] (E: fp32 22 x 64; x0 bf16 [M][64]). */
+int sseqs_enc_embed(const uint8_t* code, int64_t M, const float* E, void* x0, hipStream_t stream);
+/* dE[code[p]] += dx0[p] for codes 0..19 (dx0 fp32 [M][64]). */
+int sseqs_enc_embed_grad(const uint8_t* code, int64_t M, const float* dx0, float* dE,
+                         hipStream_t stream);
+/* e[s] = z / |z| with z = mean of h3 rows row0[s] .. + len[s] (h3 bf16
+ * [.][256]); znorm[s] = |z| (may be null). */
+int sseqs_enc_pool(const void* h3, const int64_t* row0, const int32_t* len, int64_t nseq,
+                   float* e, float* znorm, hipStream_t stream);
+/* dU3 on each sequence's rows: ((de - e (e . de)) / |z|) / len * gelu'(U3). */
+int sseqs_enc_pool_grad(const float* de, const float* e, const float* znorm, const void* U3,
+                        const int64_t* row0, const int32_t* len, int64_t nseq, void* dU3,
+                        hipStream_t stream);
+/* Squared error of cos_j = e[qi[j]] . e[ti[j]] against y[j], averaged over
+ * npair; its gradient is added to de, cos_j to cos_out (if not null), the
+ * loss to *loss (if not null).  e holds nseq rows; a pair naming a row
+ * outside 0..nseq-1 is skipped. */
+int sseqs_enc_pair_loss(const float* e, int64_t nseq, const int32_t* qi, const int32_t* ti,
+                        const float* y, int64_t npair, float* de, float* cos_out, float* loss,
+                        hipStream_t stream);
+/* AdamW step `step` (from 1) over n fp32 parameters. */
+int sseqs_enc_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1,
+                   float b2, float eps, float wd, int step, hipStream_t stream);
+/* bf16 forward and input-gradient layouts of W[t][i][o] (bwd may be null). */
+int sseqs_enc_pack_weights(const float* W, int Cin, int Cout, void* fwd, void* bwd,
+                           hipStream_t stream);
+/* Codes (0..19, 20 other) and mask 1 of database sequences s_begin + s
+ * (s < nseq) at rows row0[s] .. + min(len, max_len); other rows untouched. */
+int sseqs_enc_db_rows(const uint8_t* packed, const int64_t* seq_off, const int32_t* seq_len,
+                      int64_t s_begin, int64_t nseq, const int64_t* row0, int max_len,
+                      uint8_t* code, uint8_t* mask, hipStream_t stream);
+/* out[i] = fp8 e4m3 (OCP) of scale * e[i], n a multiple of 4. */
+int sseqs_enc_to_fp8(const float* e, int64_t n, float scale, uint8_t* out, hipStream_t stream);
+/* scores[q * ld + s] (fp16) = sum_c table[s][c] * q8[q][c] for the n rows of
+ * the fp8 table (256 bytes each) and G query rows (G a multiple of 16, at
+ * most 128), on MFMA fp8. */
+int sseqs_propose(const uint8_t* table, int64_t n, const uint8_t* q8, int G, void* scores,
+                  int64_t ld, hipStream_t stream);
+
 /* --------------------------------------------------- synthetic database -- */
 
 /* Parameters of the synthetic generator; layout shared with

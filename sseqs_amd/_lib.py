@@ -56,6 +56,21 @@ SIGNATURES = {
     "sseqs_db_sw_linear": (_i32, [_vp, _vp, _vp, _vp, _i64, _vp, _i32, _i32, _vp, _vp]),
     "sseqs_db_sw_batch": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _i32, _i32,
                                  _i32, _i32, _i32, _vp, _vp, _vp]),
+    "sseqs_enc_conv": (_i32, [_vp, _i32, _vp, _i64, _i32, _i32, _vp, _vp, _vp, _vp, _vp, _vp, _i32,
+                              _vp]),
+    "sseqs_enc_wgrad": (_i32, [_vp, _i32, _vp, _i32, _i64, _vp, _vp]),
+    "sseqs_enc_bias_grad": (_i32, [_vp, _i32, _i64, _vp, _vp]),
+    "sseqs_enc_embed": (_i32, [_vp, _i64, _vp, _vp, _vp]),
+    "sseqs_enc_embed_grad": (_i32, [_vp, _i64, _vp, _vp, _vp]),
+    "sseqs_enc_pool": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp, _vp]),
+    "sseqs_enc_pool_grad": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
+    "sseqs_enc_pair_loss": (_i32, [_vp, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "sseqs_enc_adam": (_i32, [_vp, _vp, _vp, _vp, _i64, C.c_float, C.c_float, C.c_float, C.c_float,
+                              C.c_float, _i32, _vp]),
+    "sseqs_enc_pack_weights": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp]),
+    "sseqs_enc_db_rows": (_i32, [_vp, _vp, _vp, _i64, _i64, _vp, _i32, _vp, _vp, _vp]),
+    "sseqs_enc_to_fp8": (_i32, [_vp, _i64, C.c_float, _vp, _vp]),
+    "sseqs_propose": (_i32, [_vp, _i64, _vp, _i32, _vp, _i64, _vp]),
     "sseqs_topk_ws_bytes": (_sz, [_i64]),
     "sseqs_topk": (_i32, [_vp, _i32, _i64, _i64, _i32, _vp, _vp, _sz, _vp]),
     "sseqs_topk_segments": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32,

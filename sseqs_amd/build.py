@@ -47,6 +47,38 @@ def _compile(src: Path) -> Path:
     return obj
 
 
+EXT_SRC = CSRC / "torch_ext.cpp"
+EXT_OUT = PKG / "_torch_ext.so"
+
+
+def build_torch_ext(verbose: bool = True) -> Path:
+    """The PyTorch extension module (xbit, conv, sw over the C ABI), built
+    in-tree with the host compiler against torch's headers and libraries;
+    it finds libsseqs_amd.so next to itself through its run path."""
+    import sysconfig
+
+    import torch
+    from torch.utils import cpp_extension as ce
+    abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-shared", "-fPIC", "-std=c++17",
+           f"-D_GLIBCXX_USE_CXX11_ABI={abi}", "-DTORCH_EXTENSION_NAME=_torch_ext",
+           "-DTORCH_API_INCLUDE_EXTENSION_H", "-DUSE_ROCM=1", "-D__HIP_PLATFORM_AMD__=1",
+           *[f"-I{p}" for p in ce.include_paths()], f"-I{sysconfig.get_paths()['include']}",
+           "-I/opt/rocm/include", f"-I{ROOT / 'include'}", str(EXT_SRC),
+           *[f"-L{p}" for p in ce.library_paths()], f"-L{PKG}", "-lsseqs_amd",
+           "-lc10", "-lc10_hip", "-ltorch", "-ltorch_cpu", "-ltorch_hip", "-ltorch_python",
+           "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN",
+           *[f"-Wl,-rpath,{p}" for p in ce.library_paths()],
+           "-o", str(EXT_OUT.with_suffix(".so.tmp"))]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"torch extension build failed:\n{r.stderr[-4000:]}")
+    os.replace(EXT_OUT.with_suffix(".so.tmp"), EXT_OUT)
+    if verbose:
+        print(f"built {EXT_OUT.relative_to(ROOT)}")
+    return EXT_OUT
+
+
 def build(verbose: bool = True) -> Path:
     BUILD.mkdir(exist_ok=True)
     srcs = sorted(CSRC.glob("*.hip"))
@@ -63,6 +95,7 @@ def build(verbose: bool = True) -> Path:
     os.replace(tmp, OUT)
     if verbose:
         print(f"built {OUT.relative_to(ROOT)} from {len(srcs)} sources")
+    build_torch_ext(verbose)
     return OUT
 
 

@@ -1,0 +1,642 @@
+// Learned prefilter: the sequence encoder and the candidate proposal.
+//
+// Encoder of one sequence (codes 0..19, 20 for anything else):
+//   x0[p] = E[code[p]]                              E: 22 x 64, rows 20, 21 zero
+//   h1    = GELU(conv9(x0; W1: 64 -> 128) + b1)
+//   h2    = GELU(conv9(h1; W2: 128 -> 128) + b2)
+//   h3    = GELU(conv9(h2; W3: 128 -> 256) + b3)
+//   z     = mean over positions of h3,  e = z / |z|      (256-d)
+// conv9(x; W)[p][o] = sum_{t<9} sum_i x[p + t - 4][i] * W[t][i][o], zero
+// outside the sequence.  The similarity of two sequences is e_q . e_t.
+//
+// Layout: a batch of sequences is one flat array of rows (positions) with
+// four zero "pad" rows before every sequence and after the last one, and
+// the row count rounded up to a multiple of 128 with pad rows; mask[p] = 1
+// on residue rows, 0 on pad rows.  A layer's input row p + t - 4 of a
+// residue row never leaves its sequence and its pads, so a conv layer is
+// one GEMM over all rows: C[p][o] = sum_k A[p][k] * Bt[o][k] with
+// A[p][k] = X[(p - 4) * Cin + k], k < 9 * Cin -- row p of A is the
+// contiguous window of rows p-4 .. p+4 of X -- and Bt the weights as
+// [o][t * Cin + i].  Every layer's output is zeroed on pad rows, so pads
+// stay zero for the next layer.  The backward pass over the input is the
+// same GEMM on dU (gradient before the GELU) with the weights flipped and
+// transposed, Bt'[i][t * Cout + o] = W[8 - t][i][o]; the weight gradient
+// dW[t * Cin + i][o] = sum_p A[p][t * Cin + i] * dU[p][o] reduces over rows.
+//
+// bf16 operands, fp32 accumulation on v_mfma_f32_32x32x16_bf16.  The
+// proposal scores e_q . e_t of many queries against an FP8 (e4m3) table of
+// every database sequence on v_mfma_f32_16x16x32_fp8_fp8.
+#include "common.h"
+
+namespace sseqs {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline float bf2f(__bf16 v) { return (float)v; }
+__device__ inline __bf16 f2bf(float v) { return (__bf16)v; }
+
+// GELU, tanh form, and its derivative.
+__device__ inline float gelu(float x) {
+    const float c = 0.7978845608028654f;  // sqrt(2 / pi)
+    const float u = c * (x + 0.044715f * x * x * x);
+    return 0.5f * x * (1.f + tanhf(u));
+}
+__device__ inline float gelu_grad(float x) {
+    const float c = 0.7978845608028654f;
+    const float u = c * (x + 0.044715f * x * x * x);
+    const float t = tanhf(u);
+    return 0.5f * (1.f + t) + 0.5f * x * (1.f - t * t) * c * (1.f + 3.f * 0.044715f * x * x);
+}
+
+// ------------------------------------------------------------- conv GEMM --
+//
+// Block of 256 threads (4 waves) computes a 128 x BN tile of C; wave
+// (wm, wn) the 64 x BN/2 sub-tile, as 2 x BN/64 MFMA tiles of 32 x 32.
+// K advances 32 per stage: A (128 x 32) and Bt (BN x 32) tiles go through
+// LDS (rows padded to 40 elements), the next stage's global loads are
+// issued before the current stage's MFMAs.
+//
+// MFMA 32x32x16 operands: lane (r = lane % 32, h = lane / 32) supplies
+// row r, elements 8h .. 8h+7 of K of both A and Bt; result element e of
+// lane (c = lane % 32, h) is row 8 * (e / 4) + 4 * h + (e % 4), column c.
+constexpr int GM = 128, GK = 32, GPAD = 40;
+
+enum ConvEpi {
+    EPI_FWD = 0,      // u = acc + bias; U = u (optional); Y = gelu(u) * mask
+    EPI_BWD_GELU = 1, // dUprev = acc * gelu'(Uprev) * mask
+    EPI_BWD_F32 = 2,  // dX (fp32) = acc * mask (input of layer 1)
+};
+
+struct ConvArgs {
+    const __bf16* A;     // A row p at A + (p - 4) * lda (rows -4 .. M + 3 readable)
+    int lda;             // elements between consecutive rows of A (Cin or Cout)
+    const __bf16* Bt;    // N rows of K
+    int M, N, K;
+    const float* bias;   // EPI_FWD
+    const uint8_t* mask; // [M]
+    __bf16* U;           // EPI_FWD: pre-activation out (may be null)
+    __bf16* Y;           // EPI_FWD: activation out; EPI_BWD_GELU: gradient out
+    const __bf16* Uprev; // EPI_BWD_GELU
+    float* Yf;           // EPI_BWD_F32
+};
+
+template <int BN, int EPI>
+__global__ __launch_bounds__(256) void k_conv_gemm(ConvArgs a) {
+    constexpr int NT = BN / 64;  // MFMA tiles per wave along N
+    __shared__ __align__(16) __bf16 sA[2][GM * GPAD];
+    __shared__ __align__(16) __bf16 sB[2][BN * GPAD];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = w >> 1, wn = w & 1;
+    const int r = lane & 31, h = lane >> 5;
+    const int64_t m0 = (int64_t)blockIdx.x * GM;
+    const int n0 = blockIdx.y * BN;
+
+    // global -> register staging: A tile 128 x 32 = 512 chunks of 8, B tile
+    // BN x 32 = BN * 4 chunks; thread i takes chunks i and i + 256
+    const int crow = tid >> 2, ccol = (tid & 3) * 8;
+    const __bf16* gA0 = a.A + (m0 + crow - 4) * a.lda + ccol;
+    const __bf16* gA1 = gA0 + (int64_t)64 * a.lda;
+    const __bf16* gB0 = a.Bt + (int64_t)(n0 + crow) * a.K + ccol;
+    const __bf16* gB1 = gB0 + (int64_t)64 * a.K;
+    uint4 ra0, ra1, rb0, rb1 = make_uint4(0, 0, 0, 0);
+#define CONV_GLOAD(k0)                                                     \
+    do {                                                                   \
+        ra0 = *reinterpret_cast<const uint4*>(gA0 + (k0));                 \
+        ra1 = *reinterpret_cast<const uint4*>(gA1 + (k0));                 \
+        rb0 = *reinterpret_cast<const uint4*>(gB0 + (k0));                 \
+        if (BN == 128) rb1 = *reinterpret_cast<const uint4*>(gB1 + (k0));  \
+    } while (0)
+#define CONV_SSTORE(b)                                                                   \
+    do {                                                                                 \
+        *reinterpret_cast<uint4*>(&sA[b][crow * GPAD + ccol]) = ra0;                     \
+        *reinterpret_cast<uint4*>(&sA[b][(crow + 64) * GPAD + ccol]) = ra1;              \
+        *reinterpret_cast<uint4*>(&sB[b][crow * GPAD + ccol]) = rb0;                     \
+        if (BN == 128) *reinterpret_cast<uint4*>(&sB[b][(crow + 64) * GPAD + ccol]) = rb1; \
+    } while (0)
+
+    f32x16 acc[2][NT];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+#pragma unroll
+            for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.f;
+
+    const int nk = a.K / GK;
+    CONV_GLOAD(0);
+    CONV_SSTORE(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; ++kt) {
+        const int buf = kt & 1;
+        if (kt + 1 < nk) CONV_GLOAD((kt + 1) * GK);
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            bf16x8 fa[2], fb[NT];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                fa[i] = *reinterpret_cast<const bf16x8*>(
+                    &sA[buf][(64 * wm + 32 * i + r) * GPAD + 16 * s + 8 * h]);
+#pragma unroll
+            for (int j = 0; j < NT; ++j)
+                fb[j] = *reinterpret_cast<const bf16x8*>(
+                    &sB[buf][((BN / 2) * wn + 32 * j + r) * GPAD + 16 * s + 8 * h]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < NT; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j],
+                                                                        0, 0, 0);
+        }
+        if (kt + 1 < nk) CONV_SSTORE(buf ^ 1);
+        __syncthreads();
+    }
+#undef CONV_GLOAD
+#undef CONV_SSTORE
+
+    // epilogue
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+            const int n = n0 + (BN / 2) * wn + 32 * j + r;
+#pragma unroll
+            for (int e = 0; e < 16; ++e) {
+                const int64_t m = m0 + 64 * wm + 32 * i + 8 * (e >> 2) + 4 * h + (e & 3);
+                const float v = acc[i][j][e];
+                const float mk = a.mask[m] ? 1.f : 0.f;
+                const int64_t o = m * a.N + n;
+                if constexpr (EPI == EPI_FWD) {
+                    const float u = v + a.bias[n];
+                    if (a.U) a.U[o] = f2bf(u);
+                    a.Y[o] = f2bf(gelu(u) * mk);
+                } else if constexpr (EPI == EPI_BWD_GELU) {
+                    a.Y[o] = f2bf(v * gelu_grad(bf2f(a.Uprev[o])) * mk);
+                } else {
+                    a.Yf[o] = v * mk;
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------- weight grad --
+//
+// dW[k][o] += sum_p A[p][k] * dU[p][o] over the rows p of this block's
+// range, k = t * Cin + i.  Block: 64 rows of k x 64 columns of o, rows p in
+// steps of 32 through LDS (X rows p-4 .. p+27+4 as the window source, dU
+// rows p .. p+31), MFMA with K = rows: lane (r, h) gathers 8 consecutive
+// rows of one column from LDS.  Partial sums of a block's row range are
+// added to dW (fp32) atomically.
+constexpr int WP = 32;   // rows per step
+constexpr int WROWS = 2048;  // rows per block
+
+__global__ __launch_bounds__(256) void k_conv_wgrad(const __bf16* __restrict__ X, int Cin,
+                                                    const __bf16* __restrict__ dU, int Cout,
+                                                    int64_t M, float* __restrict__ dW) {
+    // tile: k in [k0, k0 + 64) (k = t * Cin + i), o in [o0, o0 + 64); the
+    // k-columns of row p are X[(p - 4) * Cin + k0 .. + 64), contiguous
+    const int k0 = blockIdx.x * 64, o0 = blockIdx.y * 64;
+    const int64_t p_lo = (int64_t)blockIdx.z * WROWS;
+    const int64_t p_hi = min(p_lo + WROWS, M);
+    __shared__ __align__(16) __bf16 sA[WP * 64];
+    __shared__ __align__(16) __bf16 sD[WP * 64];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int r = lane & 31, h = lane >> 5;
+    const int wm = w >> 1, wn = w & 1;  // wave sub-tile: 32 k x 32 o
+    f32x16 acc;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = 0.f;
+    for (int64_t p0 = p_lo; p0 < p_hi; p0 += WP) {
+        __syncthreads();
+        {   // 32 rows x 64 columns of A and of dU, 8 elements per thread
+            const int row = tid >> 3, col = (tid & 7) * 8;
+            const int64_t p = p0 + row;
+            uint4 va = make_uint4(0, 0, 0, 0), vd = make_uint4(0, 0, 0, 0);
+            if (p < p_hi) {
+                va = *reinterpret_cast<const uint4*>(X + (p - 4) * Cin + k0 + col);
+                vd = *reinterpret_cast<const uint4*>(dU + p * Cout + o0 + col);
+            }
+            *reinterpret_cast<uint4*>(&sA[row * 64 + col]) = va;
+            *reinterpret_cast<uint4*>(&sD[row * 64 + col]) = vd;
+        }
+        __syncthreads();
+        // K of the MFMA runs over rows: lane (r, h) gathers 8 rows of one column
+#pragma unroll
+        for (int s = 0; s < WP / 16; ++s) {
+            bf16x8 fa, fb;
+#pragma unroll
+            for (int x = 0; x < 8; ++x) {
+                const int pp = 16 * s + 8 * h + x;
+                fa[x] = sA[pp * 64 + 32 * wm + r];
+                fb[x] = sD[pp * 64 + 32 * wn + r];
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa, fb, acc, 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) {
+        const int k = k0 + 32 * wm + 8 * (e >> 2) + 4 * h + (e & 3);
+        const int o = o0 + 32 * wn + r;
+        atomicAdd(&dW[(int64_t)k * Cout + o], acc[e]);
+    }
+}
+
+// db[o] += sum over rows of dU[p][o]: a block sums 1024 rows per column
+// (one thread per column of 64, four row phases), then one atomic each.
+__global__ __launch_bounds__(256) void k_colsum(const __bf16* __restrict__ dU, int C, int64_t M,
+                                                float* __restrict__ db) {
+    __shared__ float s[4][64];
+    const int c = blockIdx.y * 64 + (threadIdx.x & 63), ph = threadIdx.x >> 6;
+    const int64_t lo = (int64_t)blockIdx.x * 1024, hi = min(lo + 1024, M);
+    float acc = 0.f;
+    for (int64_t p = lo + ph; p < hi; p += 4) acc += bf2f(dU[p * C + c]);
+    s[ph][threadIdx.x & 63] = acc;
+    __syncthreads();
+    if (ph == 0) atomicAdd(&db[c], s[0][threadIdx.x] + s[1][threadIdx.x] + s[2][threadIdx.x] +
+                                       s[3][threadIdx.x]);
+}
+
+// ------------------------------------------------------------ small ops --
+
+// x0[p] = E[code[p]] (bf16 rows of 64)
+__global__ void k_embed(const uint8_t* __restrict__ code, int64_t M, const float* __restrict__ E,
+                        __bf16* __restrict__ x0) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // one thread: 8 channels
+    if (i >= M * 8) return;
+    const int64_t p = i >> 3;
+    const int c0 = (int)(i & 7) * 8;
+    const int cd = code[p];
+    bf16x8 v;
+#pragma unroll
+    for (int x = 0; x < 8; ++x) v[x] = f2bf(E[cd * 64 + c0 + x]);
+    *reinterpret_cast<bf16x8*>(x0 + p * 64 + c0) = v;
+}
+
+// dE[code][c] += dX0[p][c]: per-block sums in LDS, then one atomic per entry
+__global__ __launch_bounds__(256) void k_embed_grad(const uint8_t* __restrict__ code, int64_t M,
+                                                    const float* __restrict__ dx0,
+                                                    float* __restrict__ dE) {
+    __shared__ float s[22 * 64];
+    for (int i = threadIdx.x; i < 22 * 64; i += 256) s[i] = 0.f;
+    __syncthreads();
+    const int c = threadIdx.x & 63, sub = threadIdx.x >> 6;
+    for (int64_t p = (int64_t)blockIdx.x * 4 + sub; p < M; p += (int64_t)gridDim.x * 4)
+        atomicAdd(&s[code[p] * 64 + c], dx0[p * 64 + c]);
+    __syncthreads();
+    for (int i = threadIdx.x; i < 20 * 64; i += 256)
+        if (s[i] != 0.f) atomicAdd(&dE[i], s[i]);
+}
+
+// Sequence s occupies rows [row0[s], row0[s] + len[s]).  z = mean of h3
+// rows, e = z / |z| (|z| = 0 leaves e = 0); one block of 256 threads (one
+// per channel) per sequence.
+__global__ __launch_bounds__(256) void k_pool(const __bf16* __restrict__ h3,
+                                              const int64_t* __restrict__ row0,
+                                              const int32_t* __restrict__ len, int64_t nseq,
+                                              float* __restrict__ e, float* __restrict__ znorm) {
+    __shared__ float red[4];
+    const int c = threadIdx.x;
+    for (int64_t s = blockIdx.x; s < nseq; s += gridDim.x) {
+        const int64_t b = row0[s];
+        const int L = len[s];
+        float acc = 0.f;
+        for (int p = 0; p < L; ++p) acc += bf2f(h3[(b + p) * 256 + c]);
+        const float z = L > 0 ? acc / (float)L : 0.f;
+        float sq = z * z;
+        for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
+        if ((c & 63) == 0) red[c >> 6] = sq;
+        __syncthreads();
+        const float nrm = sqrtf(red[0] + red[1] + red[2] + red[3]);
+        e[s * 256 + c] = nrm > 0.f ? z / nrm : 0.f;
+        if (c == 0 && znorm) znorm[s] = nrm;
+        __syncthreads();
+    }
+}
+
+// Backward of the pool and of layer 3's GELU: dz = (de - e (e . de)) / |z|,
+// dU3[p][c] = dz[c] / L * gelu'(U3[p][c]) on the sequence's rows.
+__global__ __launch_bounds__(256) void k_pool_grad(const float* __restrict__ de,
+                                                   const float* __restrict__ e,
+                                                   const float* __restrict__ znorm,
+                                                   const __bf16* __restrict__ U3,
+                                                   const int64_t* __restrict__ row0,
+                                                   const int32_t* __restrict__ len, int64_t nseq,
+                                                   __bf16* __restrict__ dU3) {
+    __shared__ float red[4];
+    const int c = threadIdx.x;
+    for (int64_t s = blockIdx.x; s < nseq; s += gridDim.x) {
+        const float ev = e[s * 256 + c], dv = de[s * 256 + c];
+        float dot = ev * dv;
+        for (int off = 32; off > 0; off >>= 1) dot += __shfl_xor(dot, off, 64);
+        if ((c & 63) == 0) red[c >> 6] = dot;
+        __syncthreads();
+        const float d = red[0] + red[1] + red[2] + red[3];
+        const float nrm = znorm[s];
+        const int L = len[s];
+        const float dz = nrm > 0.f && L > 0 ? (dv - ev * d) / nrm / (float)L : 0.f;
+        const int64_t b = row0[s];
+        for (int p = 0; p < L; ++p) {
+            const int64_t o = (b + p) * 256 + c;
+            dU3[o] = f2bf(dz * gelu_grad(bf2f(U3[o])));
+        }
+        __syncthreads();
+    }
+}
+
+// Pair loss: for pair j, cos = e[qi] . e[ti], loss += (cos - y)^2 / npair;
+// de[qi] += 2 (cos - y) / npair * e[ti] and symmetrically.  One wave per
+// pair (4 channels per lane).
+__global__ __launch_bounds__(256) void k_pair_loss(const float* __restrict__ e, int64_t nseq,
+                                                   const int32_t* __restrict__ qi,
+                                                   const int32_t* __restrict__ ti,
+                                                   const float* __restrict__ y, int64_t npair,
+                                                   float* __restrict__ de, float* __restrict__ cos_out,
+                                                   float* __restrict__ loss) {
+    const int lane = threadIdx.x & 63;
+    const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= npair) return;
+    // pairs naming a sequence outside e are skipped (and counted nowhere)
+    if (qi[j] < 0 || qi[j] >= nseq || ti[j] < 0 || ti[j] >= nseq) return;
+    const float* a = e + (int64_t)qi[j] * 256;
+    const float* b = e + (int64_t)ti[j] * 256;
+    float d = 0.f;
+#pragma unroll
+    for (int x = 0; x < 4; ++x) d += a[lane * 4 + x] * b[lane * 4 + x];
+    for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
+    const float g = 2.f * (d - y[j]) / (float)npair;
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+        atomicAdd(&de[(int64_t)qi[j] * 256 + lane * 4 + x], g * b[lane * 4 + x]);
+        atomicAdd(&de[(int64_t)ti[j] * 256 + lane * 4 + x], g * a[lane * 4 + x]);
+    }
+    if (lane == 0) {
+        if (cos_out) cos_out[j] = d;
+        if (loss) atomicAdd(loss, (d - y[j]) * (d - y[j]) / (float)npair);
+    }
+}
+
+// AdamW step on fp32 parameters.
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, int64_t n, float lr, float b1, float b2, float eps,
+                       float wd, float c1, float c2) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    const float mi = b1 * m[i] + (1.f - b1) * gi;
+    const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    p[i] -= lr * ((mi / c1) / (sqrtf(vi / c2) + eps) + wd * p[i]);
+}
+
+// Weight layouts for the GEMMs from W[t][i][o] (fp32): forward
+// Bt[o][t * Cin + i], and backward-data Bt'[i][t * Cout + o] = W[8 - t][i][o].
+__global__ void k_pack_weights(const float* __restrict__ W, int Cin, int Cout,
+                               __bf16* __restrict__ fwd, __bf16* __restrict__ bwd) {
+    const int64_t n = (int64_t)9 * Cin * Cout;
+    const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= n) return;
+    const int o = (int)(idx % Cout);
+    const int i = (int)((idx / Cout) % Cin);
+    const int t = (int)(idx / ((int64_t)Cout * Cin));
+    const __bf16 v = f2bf(W[idx]);
+    fwd[(int64_t)o * 9 * Cin + t * Cin + i] = v;
+    if (bwd) bwd[(int64_t)i * 9 * Cout + (8 - t) * Cout + o] = v;
+}
+
+// ------------------------------------------------------------- table --
+
+// Flat rows of a range of database sequences: codes (0..19, 20 other,
+// 21 pad) and mask; sequence s of the range starts at row row0[s].
+__global__ void k_db_rows(const uint8_t* __restrict__ packed, const int64_t* __restrict__ off,
+                          const int32_t* __restrict__ len, int64_t s_begin, int64_t nseq,
+                          const int64_t* __restrict__ row0, int max_len,
+                          uint8_t* __restrict__ code, uint8_t* __restrict__ mask) {
+    for (int64_t s = blockIdx.x; s < nseq; s += gridDim.x) {
+        const int64_t o = off[s_begin + s];
+        const int L = min(len[s_begin + s], max_len);
+        const int64_t b = row0[s];
+        for (int p = threadIdx.x; p < L; p += blockDim.x) {
+            const int a = aa_index(packed_sym(packed, o + p));
+            code[b + p] = (uint8_t)a;
+            mask[b + p] = 1;
+        }
+    }
+}
+
+// Table rows: fp8 (e4m3) of scale * e, 256 bytes per sequence.
+__global__ void k_to_fp8(const float* __restrict__ e, int64_t n, float scale,
+                         uint8_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // 4 values per thread
+    if (i * 4 >= n) return;
+    const float4 v = *reinterpret_cast<const float4*>(e + i * 4);
+    int w = __builtin_amdgcn_cvt_pk_fp8_f32(scale * v.x, scale * v.y, 0, false);
+    w = __builtin_amdgcn_cvt_pk_fp8_f32(scale * v.z, scale * v.w, w, true);
+    *reinterpret_cast<int*>(out + i * 4) = w;
+}
+
+// ---------------------------------------------------------- proposal --
+//
+// S[q][s] = sum_c T[s][c] * Q[q][c] over the fp8 table T (n x 256) and G
+// query rows Q (G x 256, G a multiple of 16, at most 128), written as fp16
+// to S[q * ldS + s].  Block: 128 table rows (4 waves x 32 rows) against
+// all G queries held in LDS; MFMA 16x16x32 fp8: lane (r = lane % 16,
+// g = lane / 16) supplies row r, bytes 8g .. 8g+7 of the 32-byte K slice;
+// result element i of lane (c, g) is row 4g + i, column c.
+__global__ __launch_bounds__(256) void k_propose(const uint8_t* __restrict__ T, int64_t n,
+                                                 const uint8_t* __restrict__ Q, int G,
+                                                 _Float16* __restrict__ S, int64_t ldS) {
+    __shared__ __align__(16) uint8_t sQ[128 * 256];
+    for (int i = threadIdx.x; i < G * 16; i += 256)
+        reinterpret_cast<uint4*>(sQ)[i] = reinterpret_cast<const uint4*>(Q)[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r = lane & 15, g = lane >> 4;
+    for (int64_t t0 = ((int64_t)blockIdx.x * 4 + w) * 32; t0 < n;
+         t0 += (int64_t)gridDim.x * 128) {
+        // 32 table rows (two MFMA row tiles) of this wave, all 8 K-slices
+        long fa[2][8];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const int64_t row = min(t0 + 16 * i + r, n - 1);
+#pragma unroll
+            for (int s = 0; s < 8; ++s)
+                fa[i][s] = *reinterpret_cast<const long*>(T + row * 256 + 32 * s + 8 * g);
+        }
+        for (int q0 = 0; q0 < G; q0 += 16) {
+            f32x4 acc[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                const long fb = *reinterpret_cast<const long*>(&sQ[(q0 + r) * 256 + 32 * s + 8 * g]);
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(fa[i][s], fb, acc[i], 0,
+                                                                        0, 0);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; ++i) {
+#pragma unroll
+                for (int x = 0; x < 4; ++x) {
+                    const int64_t row = t0 + 16 * i + 4 * g + x;
+                    if (row < n) S[(int64_t)(q0 + r) * ldS + row] = (_Float16)acc[i][x];
+                }
+            }
+        }
+    }
+}
+
+}  // namespace sseqs
+
+using namespace sseqs;
+
+// ---------------------------------------------------------------- C ABI --
+
+extern "C" int sseqs_enc_conv(const void* A, int lda, const void* Bt, int64_t M, int N, int K,
+                              const float* bias, const uint8_t* mask, void* U, void* Y,
+                              const void* Uprev, float* Yf, int epi, hipStream_t stream) {
+    if (M <= 0 || M % GM) return fail(-1, "enc_conv: M must be a positive multiple of 128");
+    if (K <= 0 || K % GK) return fail(-1, "enc_conv: K must be a positive multiple of 32");
+    if (N % 64) return fail(-1, "enc_conv: N must be a multiple of 64");
+    if (M / GM > 0x7fffffffLL) return fail(-1, "enc_conv: too many rows");
+    ConvArgs a{static_cast<const __bf16*>(A), lda, static_cast<const __bf16*>(Bt), (int)0, N, K,
+               bias, mask, static_cast<__bf16*>(U), static_cast<__bf16*>(Y),
+               static_cast<const __bf16*>(Uprev), Yf};
+    a.M = (int)(M < 0x7fffffffLL ? M : 0x7fffffffLL);
+    const bool wide = N % 128 == 0;
+    const dim3 grid((unsigned)(M / GM), (unsigned)(N / (wide ? 128 : 64)));
+    switch (epi) {
+    case EPI_FWD:
+        if (!Y || !bias) return fail(-1, "enc_conv: forward needs Y and bias");
+        if (wide) hipLaunchKernelGGL((k_conv_gemm<128, EPI_FWD>), grid, dim3(256), 0, stream, a);
+        else hipLaunchKernelGGL((k_conv_gemm<64, EPI_FWD>), grid, dim3(256), 0, stream, a);
+        break;
+    case EPI_BWD_GELU:
+        if (!Y || !Uprev) return fail(-1, "enc_conv: backward needs Y and Uprev");
+        if (wide) hipLaunchKernelGGL((k_conv_gemm<128, EPI_BWD_GELU>), grid, dim3(256), 0, stream, a);
+        else hipLaunchKernelGGL((k_conv_gemm<64, EPI_BWD_GELU>), grid, dim3(256), 0, stream, a);
+        break;
+    case EPI_BWD_F32:
+        if (!Yf) return fail(-1, "enc_conv: backward to the input needs Yf");
+        if (wide) hipLaunchKernelGGL((k_conv_gemm<128, EPI_BWD_F32>), grid, dim3(256), 0, stream, a);
+        else hipLaunchKernelGGL((k_conv_gemm<64, EPI_BWD_F32>), grid, dim3(256), 0, stream, a);
+        break;
+    default:
+        return fail(-1, "enc_conv: unknown epilogue");
+    }
+    return check_launch("enc_conv");
+}
+
+extern "C" int sseqs_enc_wgrad(const void* X, int Cin, const void* dU, int Cout, int64_t M,
+                               float* dW, hipStream_t stream) {
+    if (M <= 0 || Cin % 64 || Cout % 64) return fail(-1, "enc_wgrad: bad sizes");
+    if (ceil_div(M, WROWS) > 65535) return fail(-1, "enc_wgrad: at most 65535 * 2048 rows");
+    const dim3 grid((unsigned)(9 * Cin / 64), (unsigned)(Cout / 64),
+                    (unsigned)ceil_div(M, WROWS));
+    hipLaunchKernelGGL(k_conv_wgrad, grid, dim3(256), 0, stream, static_cast<const __bf16*>(X),
+                       Cin, static_cast<const __bf16*>(dU), Cout, M, dW);
+    return check_launch("enc_wgrad");
+}
+
+extern "C" int sseqs_enc_bias_grad(const void* dU, int C, int64_t M, float* db,
+                                   hipStream_t stream) {
+    if (M <= 0) return 0;
+    if (C % 64) return fail(-1, "enc_bias_grad: C must be a multiple of 64");
+    hipLaunchKernelGGL(k_colsum, dim3((unsigned)ceil_div(M, 1024), (unsigned)(C / 64)), dim3(256),
+                       0, stream, static_cast<const __bf16*>(dU), C, M, db);
+    return check_launch("enc_bias_grad");
+}
+
+extern "C" int sseqs_enc_embed(const uint8_t* code, int64_t M, const float* E, void* x0,
+                               hipStream_t stream) {
+    if (M <= 0) return 0;
+    hipLaunchKernelGGL(k_embed, dim3((unsigned)ceil_div(M * 8, 256)), dim3(256), 0, stream, code,
+                       M, E, static_cast<__bf16*>(x0));
+    return check_launch("enc_embed");
+}
+
+extern "C" int sseqs_enc_embed_grad(const uint8_t* code, int64_t M, const float* dx0, float* dE,
+                                    hipStream_t stream) {
+    if (M <= 0) return 0;
+    hipLaunchKernelGGL(k_embed_grad, dim3(grid_cap(M, 4 * 64)), dim3(256), 0, stream, code, M,
+                       dx0, dE);
+    return check_launch("enc_embed_grad");
+}
+
+extern "C" int sseqs_enc_pool(const void* h3, const int64_t* row0, const int32_t* len,
+                              int64_t nseq, float* e, float* znorm, hipStream_t stream) {
+    if (nseq <= 0) return 0;
+    hipLaunchKernelGGL(k_pool, dim3(grid_cap(nseq, 1)), dim3(256), 0, stream,
+                       static_cast<const __bf16*>(h3), row0, len, nseq, e, znorm);
+    return check_launch("enc_pool");
+}
+
+extern "C" int sseqs_enc_pool_grad(const float* de, const float* e, const float* znorm,
+                                   const void* U3, const int64_t* row0, const int32_t* len,
+                                   int64_t nseq, void* dU3, hipStream_t stream) {
+    if (nseq <= 0) return 0;
+    hipLaunchKernelGGL(k_pool_grad, dim3(grid_cap(nseq, 1)), dim3(256), 0, stream, de, e, znorm,
+                       static_cast<const __bf16*>(U3), row0, len, nseq, static_cast<__bf16*>(dU3));
+    return check_launch("enc_pool_grad");
+}
+
+extern "C" int sseqs_enc_pair_loss(const float* e, int64_t nseq, const int32_t* qi,
+                                   const int32_t* ti, const float* y, int64_t npair, float* de,
+                                   float* cos_out, float* loss, hipStream_t stream) {
+    if (npair <= 0) return 0;
+    hipLaunchKernelGGL(k_pair_loss, dim3((unsigned)ceil_div(npair, 4)), dim3(256), 0, stream, e,
+                       nseq, qi, ti, y, npair, de, cos_out, loss);
+    return check_launch("enc_pair_loss");
+}
+
+extern "C" int sseqs_enc_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr,
+                              float b1, float b2, float eps, float wd, int step,
+                              hipStream_t stream) {
+    if (n <= 0) return 0;
+    if (step < 1) return fail(-1, "enc_adam: step counts from 1");
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    hipLaunchKernelGGL(k_adam, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, p, g, m,
+                       v, n, lr, b1, b2, eps, wd, c1, c2);
+    return check_launch("enc_adam");
+}
+
+extern "C" int sseqs_enc_pack_weights(const float* W, int Cin, int Cout, void* fwd, void* bwd,
+                                      hipStream_t stream) {
+    const int64_t n = (int64_t)9 * Cin * Cout;
+    hipLaunchKernelGGL(k_pack_weights, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, W,
+                       Cin, Cout, static_cast<__bf16*>(fwd), static_cast<__bf16*>(bwd));
+    return check_launch("enc_pack_weights");
+}
+
+extern "C" int sseqs_enc_db_rows(const uint8_t* packed, const int64_t* seq_off,
+                                 const int32_t* seq_len, int64_t s_begin, int64_t nseq,
+                                 const int64_t* row0, int max_len, uint8_t* code, uint8_t* mask,
+                                 hipStream_t stream) {
+    if (nseq <= 0) return 0;
+    hipLaunchKernelGGL(k_db_rows, dim3(grid_cap(nseq, 1)), dim3(256), 0, stream, packed, seq_off,
+                       seq_len, s_begin, nseq, row0, max_len, code, mask);
+    return check_launch("enc_db_rows");
+}
+
+extern "C" int sseqs_enc_to_fp8(const float* e, int64_t n, float scale, uint8_t* out,
+                                hipStream_t stream) {
+    if (n <= 0) return 0;
+    if (n % 4) return fail(-1, "enc_to_fp8: n must be a multiple of 4");
+    hipLaunchKernelGGL(k_to_fp8, dim3((unsigned)ceil_div(n / 4, 256)), dim3(256), 0, stream, e, n,
+                       scale, out);
+    return check_launch("enc_to_fp8");
+}
+
+extern "C" int sseqs_propose(const uint8_t* table, int64_t n, const uint8_t* q8, int G,
+                             void* scores, int64_t ld, hipStream_t stream) {
+    if (n <= 0) return 0;
+    if (G <= 0 || G % 16 || G > 128) return fail(-1, "propose: G must be 16..128, a multiple of 16");
+    if (ld < n) return fail(-1, "propose: ld smaller than n");
+    const unsigned grid = (unsigned)(ceil_div(n, 128) < 8192 ? ceil_div(n, 128) : 8192);
+    hipLaunchKernelGGL(k_propose, dim3(grid), dim3(256), 0, stream, table, n, q8, G,
+                       static_cast<_Float16*>(scores), ld);
+    return check_launch("propose");
+}

@@ -1,0 +1,508 @@
+"""Learned prefilter: a convolutional sequence encoder, an FP8 embedding table
+of the whole database and an MFMA proposal of candidates for exact SW.
+
+The encoder (kernels in csrc/learned.hip): residue embedding 20 -> 64, three
+conv1d layers of width 9 (64 -> 128 -> 128 -> 256) with GELU, mean pool over
+positions and L2 normalisation; the similarity of a query and a database
+sequence is the cosine of their embeddings.  It is trained on (query,
+target, normalised SW score) pairs -- targets are planted homologs and
+background sequences of a synthetic database, the score is the affine SW
+score (reference stripe rule, 11/1) divided by the query's score against
+itself -- with hand-written forward and backward kernels (bf16 operands,
+fp32 accumulation) and AdamW, minimising the squared error between cosine
+and score.
+
+Every database sequence is embedded once into an HBM-resident table of
+256 FP8 (e4m3) values per sequence (51.2 GB for 200 M sequences).  A batch
+of queries is embedded the same way and the proposal GEMM
+(``sseqs_propose``, MFMA fp8) scores it against the whole table; each
+query's top ``top_prefilter`` go to exact affine SW, which stays the judge.
+``Proposer`` plugs this into ``multi.MultiSearch`` (and so into the serving
+engine); its ``calibrate`` measures Recall@4096 against exhaustive SW on
+sample queries, and when that falls below the gate (0.99) the proposer
+falls back to the plain prefilter.
+
+There is no counterpart in the reference, which only has the FFT prefilter
+(sseqs/msa.py:270-310); the candidates this proposes replace that
+prefilter's stage-1 selection.
+"""
+from __future__ import annotations
+
+import math
+import time
+from dataclasses import dataclass
+from pathlib import Path
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call, ptr, stream_of
+from .query import AA_MAP
+
+WEIGHTS_PATH = Path(__file__).resolve().parent / "encoder_weights.npz"
+C0, C1, C2, C3 = 64, 128, 128, 256
+DIM = C3
+FP8_SCALE = 16.0
+PAD_CODE = 21
+# parameter blocks of the flat fp32 vector: name -> shape
+SHAPES = [("E", (22, C0)), ("W1", (9, C0, C1)), ("b1", (C1,)), ("W2", (9, C1, C2)),
+          ("b2", (C2,)), ("W3", (9, C2, C3)), ("b3", (C3,))]
+
+
+def codes_of(seq: str) -> np.ndarray:
+    """Encoder codes: residues 0..19, 20 for any other letter."""
+    return np.array([AA_MAP.get(c, 20) for c in seq], dtype=np.uint8)
+
+
+@dataclass
+class Rows:
+    """Flat row layout of a batch of sequences (four pad rows before each
+    sequence and after the last, total rounded up to a multiple of 128)."""
+
+    M: int
+    row0: torch.Tensor   # int64 [nseq] first row of each sequence
+    length: torch.Tensor  # int32 [nseq]
+    row0_h: np.ndarray
+    len_h: np.ndarray
+
+    @staticmethod
+    def plan(lengths: np.ndarray, device) -> "Rows":
+        ln = np.asarray(lengths, dtype=np.int64)
+        row0 = (4 + np.concatenate([[0], np.cumsum(ln + 4)[:-1]]) if ln.size
+                else np.zeros(0, np.int64))
+        M = int((row0[-1] + ln[-1] + 4) if ln.size else 128)
+        M = (M + 127) // 128 * 128
+        return Rows(M, torch.from_numpy(row0.astype(np.int64)).to(device),
+                    torch.from_numpy(ln.astype(np.int32)).to(device), row0.astype(np.int64),
+                    ln.astype(np.int32))
+
+
+class Encoder:
+    """Encoder weights on the device, with forward (and, for training,
+    backward and AdamW) over batches of sequences."""
+
+    def __init__(self, device="cuda", seed: int = 0, weights: dict | None = None):
+        self.dev = torch.device(device)
+        sizes = [int(np.prod(s)) for _, s in SHAPES]
+        self.offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        self.n_params = int(self.offsets[-1])
+        self.p = torch.zeros(self.n_params, dtype=torch.float32, device=self.dev)
+        if weights is None:
+            rng = np.random.default_rng(seed)
+            init = {}
+            for name, shape in SHAPES:
+                if name == "E":
+                    v = rng.normal(0.0, 1.0, shape)
+                    v[20:] = 0.0
+                elif name.startswith("W"):
+                    fan_in = shape[0] * shape[1]
+                    v = rng.normal(0.0, math.sqrt(2.0 / fan_in), shape)
+                else:
+                    v = np.zeros(shape)
+                init[name] = v
+            weights = init
+        self.load_state(weights)
+        self.packed = {}
+        self._pack()
+        self.g = self.m = self.v = None
+        self.step_count = 0
+
+    # ----------------------------------------------------------- params --
+    def view(self, t: torch.Tensor, name: str) -> torch.Tensor:
+        i = [n for n, _ in SHAPES].index(name)
+        return t[self.offsets[i]: self.offsets[i + 1]].view(SHAPES[i][1])
+
+    def load_state(self, weights: dict) -> None:
+        for name, shape in SHAPES:
+            v = np.asarray(weights[name], dtype=np.float32).reshape(shape)
+            self.view(self.p, name).copy_(torch.from_numpy(v))
+
+    def state(self) -> dict:
+        return {name: self.view(self.p, name).cpu().numpy() for name, _ in SHAPES}
+
+    def save(self, path=WEIGHTS_PATH) -> None:
+        np.savez(path, **self.state())
+
+    @staticmethod
+    def load(path=WEIGHTS_PATH, device="cuda") -> "Encoder":
+        with np.load(path) as f:  # arrays only (allow_pickle stays off)
+            return Encoder(device, weights={k: f[k] for k in f.files})
+
+    def _pack(self) -> None:
+        """bf16 weight layouts of the forward and backward GEMMs."""
+        st = stream_of(self.dev)
+        for name, cin, cout in (("W1", C0, C1), ("W2", C1, C2), ("W3", C2, C3)):
+            if name not in self.packed:
+                self.packed[name] = (
+                    torch.empty(9 * cin * cout, dtype=torch.bfloat16, device=self.dev),
+                    torch.empty(9 * cin * cout, dtype=torch.bfloat16, device=self.dev))
+            f, b = self.packed[name]
+            call("sseqs_enc_pack_weights", ptr(self.view(self.p, name)), cin, cout, ptr(f), ptr(b),
+                 st)
+
+    # ---------------------------------------------------------- buffers --
+    def _buffers(self, M: int, train: bool) -> dict:
+        """Activations with four zero margin rows at both ends (the conv
+        windows read rows -4 .. M + 3)."""
+        dev = self.dev
+        z = lambda C: torch.zeros((M + 8, C), dtype=torch.bfloat16, device=dev)
+        b = {"x0": z(C0), "h1": z(C1), "h2": z(C2),
+             "h3": torch.empty((M, C3), dtype=torch.bfloat16, device=dev)}
+        if train:
+            b.update({"u1": torch.empty((M, C1), dtype=torch.bfloat16, device=dev),
+                      "u2": torch.empty((M, C2), dtype=torch.bfloat16, device=dev),
+                      "u3": torch.empty((M, C3), dtype=torch.bfloat16, device=dev),
+                      "du3": z(C3), "du2": z(C2), "du1": z(C1),
+                      "dx0": torch.empty((M, C0), dtype=torch.float32, device=dev)})
+        return b
+
+    @staticmethod
+    def _row(t: torch.Tensor) -> int:
+        """Pointer to row 0 of a buffer with four margin rows."""
+        return t.data_ptr() + 4 * t.shape[1] * t.element_size()
+
+    def _conv(self, A, lda, Bt, M, N, K, bias, mask, U, Y, Uprev, Yf, epi, st) -> None:
+        call("sseqs_enc_conv", A, lda, ptr(Bt), M, N, K, bias, ptr(mask), U, Y, Uprev, Yf, epi,
+             st)
+
+    # ---------------------------------------------------------- forward --
+    def forward(self, code: torch.Tensor, mask: torch.Tensor, rows: Rows, train: bool = False,
+                buf: dict | None = None):
+        """Embeddings e (float32 [nseq, 256], unit norm) of the sequences laid
+        out in ``rows`` (codes and mask per row).  Returns (e, znorm, buf)."""
+        st = stream_of(self.dev)
+        M = rows.M
+        b = buf if buf is not None else self._buffers(M, train)
+        call("sseqs_enc_embed", ptr(code), M, ptr(self.view(self.p, "E")), self._row(b["x0"]), st)
+        layers = (("W1", "b1", "x0", "h1", "u1", C0, C1), ("W2", "b2", "h1", "h2", "u2", C1, C2),
+                  ("W3", "b3", "h2", "h3", "u3", C2, C3))
+        for wn, bn, xin, yout, uout, cin, cout in layers:
+            yp = ptr(b[yout]) if yout == "h3" else self._row(b[yout])
+            self._conv(self._row(b[xin]), cin, self.packed[wn][0], M, cout, 9 * cin,
+                       ptr(self.view(self.p, bn)), mask, ptr(b[uout]) if train else None, yp, None,
+                       None, 0, st)
+        n = int(rows.row0.numel())
+        e = torch.empty((n, DIM), dtype=torch.float32, device=self.dev)
+        znorm = torch.empty(n, dtype=torch.float32, device=self.dev)
+        call("sseqs_enc_pool", ptr(b["h3"]), ptr(rows.row0), ptr(rows.length), n, ptr(e),
+             ptr(znorm), st)
+        return e, znorm, b
+
+    # --------------------------------------------------------- backward --
+    def backward(self, de: torch.Tensor, e: torch.Tensor, znorm: torch.Tensor, code, mask,
+                 rows: Rows, b: dict) -> None:
+        """Gradients of the loss whose gradient with respect to e is ``de``,
+        accumulated into self.g."""
+        st = stream_of(self.dev)
+        M = rows.M
+        if self.g is None:
+            self.g = torch.zeros_like(self.p)
+        g = self.g
+        n = int(rows.row0.numel())
+        b["du3"].zero_()
+        call("sseqs_enc_pool_grad", ptr(de), ptr(e), ptr(znorm), ptr(b["u3"]), ptr(rows.row0),
+             ptr(rows.length), n, self._row(b["du3"]), st)
+        # layer 3
+        call("sseqs_enc_bias_grad", self._row(b["du3"]), C3, M, ptr(self.view(g, "b3")), st)
+        call("sseqs_enc_wgrad", self._row(b["h2"]), C2, self._row(b["du3"]), C3, M,
+             ptr(self.view(g, "W3")), st)
+        self._conv(self._row(b["du3"]), C3, self.packed["W3"][1], M, C2, 9 * C3, None, mask, None,
+                   self._row(b["du2"]), ptr(b["u2"]), None, 1, st)
+        # layer 2
+        call("sseqs_enc_bias_grad", self._row(b["du2"]), C2, M, ptr(self.view(g, "b2")), st)
+        call("sseqs_enc_wgrad", self._row(b["h1"]), C1, self._row(b["du2"]), C2, M,
+             ptr(self.view(g, "W2")), st)
+        self._conv(self._row(b["du2"]), C2, self.packed["W2"][1], M, C1, 9 * C2, None, mask, None,
+                   self._row(b["du1"]), ptr(b["u1"]), None, 1, st)
+        # layer 1 and the embedding
+        call("sseqs_enc_bias_grad", self._row(b["du1"]), C1, M, ptr(self.view(g, "b1")), st)
+        call("sseqs_enc_wgrad", self._row(b["x0"]), C0, self._row(b["du1"]), C1, M,
+             ptr(self.view(g, "W1")), st)
+        self._conv(self._row(b["du1"]), C1, self.packed["W1"][1], M, C0, 9 * C1, None, mask, None,
+                   None, None, ptr(b["dx0"]), 2, st)
+        call("sseqs_enc_embed_grad", ptr(code), M, ptr(b["dx0"]), ptr(self.view(g, "E")), st)
+
+    def adam_step(self, lr: float, wd: float = 0.0, b1: float = 0.9, b2: float = 0.999,
+                  eps: float = 1e-8) -> None:
+        if self.m is None:
+            self.m = torch.zeros_like(self.p)
+            self.v = torch.zeros_like(self.p)
+        self.step_count += 1
+        call("sseqs_enc_adam", ptr(self.p), ptr(self.g), ptr(self.m), ptr(self.v), self.n_params,
+             float(lr), float(b1), float(b2), float(eps), float(wd), self.step_count,
+             stream_of(self.dev))
+        self._pack()
+        self.g.zero_()
+
+    # ---------------------------------------------------------- helpers --
+    def rows_of_strings(self, seqs: list[str]):
+        """(code, mask, rows) of host strings."""
+        lens = np.array([len(s) for s in seqs], dtype=np.int64)
+        rows = Rows.plan(lens, self.dev)
+        code = np.full(rows.M, PAD_CODE, dtype=np.uint8)
+        mask = np.zeros(rows.M, dtype=np.uint8)
+        for s, r0 in zip(seqs, rows.row0_h):
+            c = codes_of(s)
+            code[r0: r0 + c.size] = c
+            mask[r0: r0 + c.size] = 1
+        return (torch.from_numpy(code).to(self.dev), torch.from_numpy(mask).to(self.dev), rows)
+
+    def rows_of_db(self, db, ids: np.ndarray, max_len: int = 2048):
+        """(code, mask, rows) of database sequences ``ids`` (host int64)."""
+        ids = np.asarray(ids, dtype=np.int64)
+        lens = np.minimum(db.seq_len[torch.from_numpy(ids).to(db.device)].cpu().numpy(), max_len)
+        rows = Rows.plan(lens, self.dev)
+        code = torch.full((rows.M,), PAD_CODE, dtype=torch.uint8, device=self.dev)
+        mask = torch.zeros(rows.M, dtype=torch.uint8, device=self.dev)
+        d_ids = torch.from_numpy(ids).to(db.device)
+        off = db.seq_off[d_ids].contiguous()
+        ln = db.seq_len[d_ids].contiguous()
+        call("sseqs_enc_db_rows", ptr(db.packed), ptr(off), ptr(ln), 0, ids.size, ptr(rows.row0),
+             int(max_len), ptr(code), ptr(mask), stream_of(self.dev))
+        return code, mask, rows
+
+    def embed_strings(self, seqs: list[str]) -> torch.Tensor:
+        code, mask, rows = self.rows_of_strings(seqs)
+        return self.forward(code, mask, rows)[0]
+
+
+# ------------------------------------------------------------ training --
+
+@dataclass
+class TrainingSet:
+    """(query, target, y) pairs over a synthetic training database."""
+
+    db: object
+    queries: list[str]
+    q_ids: np.ndarray     # int32 [npair] query index
+    t_ids: np.ndarray     # int64 [npair] database id of the target
+    y: np.ndarray         # float32 [npair] SW(q, t) / SW(q, q)
+
+
+def make_training_set(n_db: int, n_queries: int, randoms_per_query: int, seed: int = 11,
+                      device="cuda", lo: int = 50, hi: int = 1000,
+                      hom_period: int = 2) -> TrainingSet:
+    """Synthetic training pairs: a database of n_db sequences in which every
+    hom_period-th is a planted homolog of one of n_queries queries (lengths
+    uniform in [lo, hi]; 30-90 % identity, indels), each query paired with
+    all its homologs and with randoms_per_query background sequences, scored
+    by affine SW (11/1, reference stripe rule) and normalised by the
+    query's score against itself."""
+    from . import synth
+    from .db import SeqDB
+    from .multi import MultiSearch
+    from .sw import sw
+    rng = np.random.default_rng(seed)
+    lens = rng.integers(lo, hi + 1, size=n_queries)
+    queries = [synth.make_query(int(L), 100_000 + i, seed=seed) for i, L in enumerate(lens)]
+    db = SeqDB.synthetic(n_db, queries, seed=seed, device=device, hom_period=hom_period)
+    hq = db.extra["hom_query"].cpu().numpy()
+    bg = np.nonzero(hq < 0)[0]
+    order = np.argsort(hq, kind="stable")
+    hq_sorted = hq[order]
+    starts = np.searchsorted(hq_sorted, np.arange(n_queries))
+    ends = np.searchsorted(hq_sorted, np.arange(n_queries), side="right")
+    cand, qidx = [], []
+    for i in range(n_queries):
+        hom = order[starts[i]:ends[i]]
+        rnd = rng.choice(bg, size=randoms_per_query, replace=False)
+        c = np.sort(np.concatenate([hom, rnd]))
+        cand.append(c)
+        qidx.append(np.full(c.size, i, np.int32))
+    # exact SW of every pair with the batched SW queue
+    ms = MultiSearch(db, queries, top_prefilter=1, top_hits=1, backtrack=False, tuning={})
+    counts = np.array([c.size for c in cand], dtype=np.int64)
+    c_all = np.concatenate(cand)
+    ms.kc = int(counts.max())
+    ms.cand = torch.from_numpy(c_all).to(db.device)
+    ms.cs = torch.empty(c_all.size, dtype=torch.int32, device=db.device)
+    ms.c_off = torch.from_numpy(np.concatenate([[0], np.cumsum(counts)])).to(db.device)
+    ms.set_sw_tuning({})
+    # per-query candidate counts differ: items past a query's list are empty
+    ms._sw()
+    scores = ms.cs.cpu().numpy().astype(np.float32)
+    selfs = np.array([float(sw(q, [q])[0]) for q in queries], dtype=np.float32)
+    q_all = np.concatenate(qidx)
+    y = np.clip(scores / np.maximum(selfs[q_all], 1.0), 0.0, 1.0).astype(np.float32)
+    return TrainingSet(db, queries, q_all, c_all, y)
+
+
+def train(enc: Encoder, ts: TrainingSet, steps: int, queries_per_step: int = 32,
+          pairs_per_query: int = 32, lr: float = 1e-3, seed: int = 0, log=None,
+          time_budget_s: float | None = None) -> list[float]:
+    """AdamW on the squared error between cosine and normalised SW score.
+    Each step takes queries_per_step queries and pairs_per_query of each
+    query's pairs (half homologs, half background where available)."""
+    rng = np.random.default_rng(seed)
+    nq = len(ts.queries)
+    by_q = [np.nonzero(ts.q_ids == i)[0] for i in range(nq)]
+    losses = []
+    t0 = time.time()
+    for step in range(steps):
+        qs = rng.choice(nq, size=min(queries_per_step, nq), replace=False)
+        pair_idx = np.concatenate([rng.choice(by_q[i], size=min(pairs_per_query, by_q[i].size),
+                                              replace=False) for i in qs])
+        tids = ts.t_ids[pair_idx]
+        # sequences of the step: the queries, then the targets
+        qcode, qmask, qrows = enc.rows_of_strings([ts.queries[i] for i in qs])
+        tcode, tmask, trows = enc.rows_of_db(ts.db, tids)
+        code = torch.cat([qcode, tcode])
+        mask = torch.cat([qmask, tmask])
+        rows = Rows(qrows.M + trows.M, torch.cat([qrows.row0, trows.row0 + qrows.M]),
+                    torch.cat([qrows.length, trows.length]),
+                    np.concatenate([qrows.row0_h, trows.row0_h + qrows.M]),
+                    np.concatenate([qrows.len_h, trows.len_h]))
+        e, znorm, buf = enc.forward(code, mask, rows, train=True)
+        pos = {q: k for k, q in enumerate(qs)}
+        qi = torch.from_numpy(np.array([pos[q] for q in ts.q_ids[pair_idx]], np.int32)).to(enc.dev)
+        ti = torch.arange(len(qs), len(qs) + tids.size, dtype=torch.int32, device=enc.dev)
+        y = torch.from_numpy(ts.y[pair_idx]).to(enc.dev)
+        de = torch.zeros_like(e)
+        loss = torch.zeros(1, dtype=torch.float32, device=enc.dev)
+        call("sseqs_enc_pair_loss", ptr(e), int(e.shape[0]), ptr(qi), ptr(ti), ptr(y), tids.size,
+             ptr(de), None, ptr(loss), stream_of(enc.dev))
+        enc.backward(de, e, znorm, code, mask, rows, buf)
+        warm = min(1.0, (step + 1) / 200)
+        enc.adam_step(lr * warm)
+        if step % 50 == 0 or step == steps - 1:
+            losses.append(float(loss.item()))
+            if log:
+                log(f"[train] step {step} loss {losses[-1]:.5f} ({time.time() - t0:.1f} s)")
+        if time_budget_s is not None and time.time() - t0 > time_budget_s:
+            break
+    return losses
+
+
+# --------------------------------------------------------------- table --
+
+class EmbeddingTable:
+    """FP8 embeddings of every sequence of a resident database (index
+    order), built once by the encoder in chunks of at most ``rows`` rows
+    (residues plus four pad rows per sequence)."""
+
+    def __init__(self, enc: Encoder, db, rows: int = 48_000_000, max_len: int = 2048, log=None):
+        self.enc, self.db = enc, db
+        self.n = db.n
+        self.table = torch.empty((db.n, DIM), dtype=torch.uint8, device=db.device)
+        t0 = time.time()
+        st = stream_of(db.device)
+        # chunk boundaries: consecutive sequences whose rows fit `rows`
+        need = np.minimum(db.seq_len.cpu().numpy().astype(np.int64), max_len) + 4
+        cum = np.cumsum(need)
+        bounds = [0]
+        while bounds[-1] < db.n:
+            base = cum[bounds[-1] - 1] if bounds[-1] else 0
+            nxt = int(np.searchsorted(cum, base + rows, side="right"))
+            bounds.append(max(nxt, bounds[-1] + 1))
+        bounds[-1] = db.n
+        Mcap = int(max(cum[e - 1] - (cum[b - 1] if b else 0) for b, e in zip(bounds, bounds[1:])))
+        Mcap = (Mcap + 4 + 127) // 128 * 128
+        bufs = enc._buffers(Mcap, train=False)
+        self.rows_total = int(cum[-1]) if db.n else 0
+        for ci, (b, e_) in enumerate(zip(bounds, bounds[1:])):
+            code, mask, rws = enc.rows_of_db(db, np.arange(b, e_, dtype=np.int64), max_len)
+            view = {}
+            for k, t in bufs.items():
+                if k in ("x0", "h1", "h2"):
+                    view[k] = t[: rws.M + 8]
+                    t[rws.M + 4: rws.M + 8].zero_()  # the margin after this chunk's rows
+                else:
+                    view[k] = t[: rws.M]
+            e, _, _ = enc.forward(code, mask, rws, buf=view)
+            call("sseqs_enc_to_fp8", ptr(e), e.numel(), FP8_SCALE,
+                 self.table.data_ptr() + b * DIM, st)
+            if log and ci % 50 == 0:
+                log(f"[table] {e_}/{db.n} sequences ({time.time() - t0:.1f} s)")
+        torch.cuda.synchronize(db.device)
+        self.build_s = time.time() - t0
+        del bufs
+
+    @property
+    def nbytes(self) -> int:
+        return self.table.numel()
+
+    @property
+    def flops(self) -> float:
+        """Encoder MFMA flops of the build: 2 * 9 * (64*128 + 128*128 +
+        128*256) per row."""
+        return 2.0 * 9 * (C0 * C1 + C1 * C2 + C2 * C3) * self.rows_total
+
+
+# ------------------------------------------------------------ proposer --
+
+class Proposer:
+    """Candidate selection for multi.MultiSearch from the embedding table:
+    per query, the top kc sequences by e_q . e_t (FP8 MFMA), in ascending
+    index order.  ``use_learned`` is False after a failed calibration, and
+    then the plain prefilter selects."""
+
+    def __init__(self, table: EmbeddingTable, group: int = 64, gate: float = 0.99):
+        self.table = table
+        self.group = group
+        self.gate = gate
+        self.use_learned = True
+        self.calibration: dict | None = None
+        self._scores = None
+
+    def scores_for(self, queries: list[str]) -> torch.Tensor:
+        """Proposal scores (fp16 [len(queries), n]) of up to 128 queries."""
+        tab = self.table
+        dev = tab.table.device
+        st = stream_of(dev)
+        e = tab.enc.embed_strings(queries)
+        Gp = (len(queries) + 15) // 16 * 16
+        if Gp > 128:
+            raise ValueError("at most 128 queries per call")
+        q = torch.zeros((Gp, DIM), dtype=torch.float32, device=dev)
+        q[: len(queries)] = e
+        q8 = torch.empty((Gp, DIM), dtype=torch.uint8, device=dev)
+        call("sseqs_enc_to_fp8", ptr(q), q.numel(), FP8_SCALE, ptr(q8), st)
+        out = torch.empty((Gp, tab.n), dtype=torch.float16, device=dev)
+        call("sseqs_propose", ptr(tab.table), tab.n, ptr(q8), Gp, ptr(out), tab.n, st)
+        return out[: len(queries)]
+
+    def select(self, ms) -> None:
+        if not self.use_learned:
+            saved = ms.proposer
+            ms.proposer = None
+            try:
+                ms._prefilter_select()
+            finally:
+                ms.proposer = saved
+            return
+        tab = self.table
+        enc = tab.enc
+        st = stream_of(ms.dev)
+        e = enc.embed_strings(ms.texts)
+        G = self.group
+        if self._scores is None or self._scores.shape[1] != tab.n:
+            self._scores = torch.empty((G, tab.n), dtype=torch.float16, device=ms.dev)
+        sel_ws = ms.sel_ws
+        for g0 in range(0, ms.nq, G):
+            g1 = min(ms.nq, g0 + G)
+            Gp = (g1 - g0 + 15) // 16 * 16
+            q = torch.zeros((Gp, DIM), dtype=torch.float32, device=ms.dev)
+            q[: g1 - g0] = e[g0:g1]
+            q8 = torch.empty((Gp, DIM), dtype=torch.uint8, device=ms.dev)
+            call("sseqs_enc_to_fp8", ptr(q), q.numel(), FP8_SCALE, ptr(q8), st)
+            call("sseqs_propose", ptr(tab.table), tab.n, ptr(q8), Gp, ptr(self._scores), tab.n, st)
+            for j in range(g1 - g0):
+                i = g0 + j
+                call("sseqs_topk", self._scores.data_ptr() + 2 * j * tab.n, _lib.F16, tab.n,
+                     ms.kc, 0, ms.cand.data_ptr() + 8 * i * ms.kc, ptr(sel_ws), sel_ws.numel(),
+                     st)
+
+
+def exhaustive_top(db, query: str, k: int, chunk: int = 25_000_000) -> tuple[int, torch.Tensor]:
+    """k-th best affine SW score of the query over the whole database, and
+    every score (int32, index order)."""
+    from .query import prepare
+    from .search import sw_affine_ids
+    q = prepare(query, device=db.device)
+    allsc = torch.empty(db.n, dtype=torch.int32, device=db.device)
+    for b in range(0, db.n, chunk):
+        ids = torch.arange(b, min(db.n, b + chunk), device=db.device)
+        allsc[b: b + ids.numel()] = sw_affine_ids(db, q, ids)
+    kth = int(torch.topk(allsc, k).values[-1].item())
+    return kth, allsc

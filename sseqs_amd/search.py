@@ -163,8 +163,9 @@ def backtrack_hits(db: SeqDB, q: PreparedQuery, ids: torch.Tensor,
     W = ws.width
     dev = db.device
     st = stream_of(dev)
-    call("sseqs_db_gather", ptr(db.packed), ptr(db.seq_off), ptr(db.seq_len),
-         ptr(ids.to(torch.int64).contiguous()), n, W, ptr(ws.rows), ptr(ws.lens), st)
+    ids = ids.to(torch.int64).contiguous()
+    call("sseqs_db_gather", ptr(db.packed), ptr(db.seq_off), ptr(db.seq_len), ptr(ids), n, W,
+         ptr(ws.rows), ptr(ws.lens), st)
     cell_off = ws.t_off[:n] * Lq
     total = Lq * n * W
     call("sseqs_backtrack", ptr(q.qbytes), Lq, ptr(ws.rows), ptr(ws.t_off),
@@ -439,8 +440,9 @@ def _resident_batches(db: SeqDB, query: str, q: PreparedQuery, plan, top_fft: in
     if n2:
         aff_off = (plan.sw_off[ids2] + 1).contiguous()
         aff_len = plan.sw_len[ids2].contiguous()
-        call("sseqs_db_sw_affine", ptr(db.packed), ptr(aff_off), ptr(aff_len),
-             ptr(torch.arange(n2, device=dev)), n2, ptr(q.codes), Lq_sw, 11, 1, ptr(sw2), st)
+        ident = torch.arange(n2, device=dev)
+        call("sseqs_db_sw_affine", ptr(db.packed), ptr(aff_off), ptr(aff_len), ptr(ident), n2,
+             ptr(q.codes), Lq_sw, 11, 1, ptr(sw2), st)
     # top top_sw_affine by affine score (sorted); the hits' fields
     k3 = np.full(b2.size, top_sw_affine, dtype=np.int64)
     pos3, _ = ops.topk_segments(sw2, base2, k2o, k3, sorted=True)

@@ -9,8 +9,10 @@ Routes, as in server.py:30-110,133-169:
   <sha256>.a3m``) are searched with one ``msa()`` call. Each a3m gets '@'
   replaced by 'X' and its first line replaced by the request's header; the
   a3m files are joined with ``b"\\x00\\n"`` and packed as ``uniref.a3m``
-  (``pair.a3m`` on the pair route) plus an empty
-  ``bfd.mgnify30.metaeuk30.smag30.a3m`` into a tar.gz. The route answers
+  (``pair.a3m`` on the pair route) plus
+  ``bfd.mgnify30.metaeuk30.smag30.a3m`` (empty, as in server.py, unless an
+  environmental database is configured: ``ENVDB`` / ``env_msa_kwargs``)
+  into a tar.gz. The route answers
   ``{"status": "COMPLETE", "id": <tar id>}``.
 * ``GET /ticket/{tar_id}``: ``{"status": "COMPLETE" | "UNKNOWN", "id": ...}``,
   the status poll of the ColabFold client API (absent from server.py).
@@ -53,7 +55,8 @@ def _sha(s: str) -> str:
 
 
 def create_app(msa_fn: Callable | None = None, root: str | os.PathLike = ".",
-               chunks: int | None = None, msa_kwargs: dict | None = None, engine=None):
+               chunks: int | None = None, msa_kwargs: dict | None = None, engine=None,
+               env_msa_kwargs: dict | None = None):
     """FastAPI app serving the reference routes.
 
     msa_fn: the search, called as ``msa_fn(queries, filenames, **options)``;
@@ -69,6 +72,12 @@ def create_app(msa_fn: Callable | None = None, root: str | os.PathLike = ".",
     cascade (its top_prefilter and top_hits) replaces the per-batch
     cascade options of the request, and its cache folders are named
     ``engine_<top_prefilter>_<top_hits>`` so the two never mix.
+
+    env_msa_kwargs: keyword arguments (for example ``db=`` an environmental
+    database) of a second search of every ticket's sequences, whose a3m
+    texts fill the ``bfd.mgnify30.metaeuk30.smag30.a3m`` member the same
+    way ``uniref.a3m`` is filled (server.py writes it empty, which is what
+    happens without this argument).
     """
     from fastapi import FastAPI, HTTPException, Request
     from fastapi.concurrency import run_in_threadpool
@@ -120,10 +129,12 @@ def create_app(msa_fn: Callable | None = None, root: str | os.PathLike = ".",
                     if inflight.get(f) is fut and fut.done():
                         del inflight[f]
 
-    def search(queries: list[str], filenames: list[str], **opts) -> None:
-        if engine is not None:
+    def search(queries: list[str], filenames: list[str], kwargs: dict | None = None,
+               **opts) -> None:
+        if engine is not None and kwargs is None:
             engine_search(queries, filenames)
             return
+        kw = extra if kwargs is None else kwargs
         """Search the queries whose a3m is not yet in the cache.  The check
         is repeated under the lock, so a sequence that two requests both
         found missing is searched once; each a3m is written to a file of its
@@ -137,7 +148,7 @@ def create_app(msa_fn: Callable | None = None, root: str | os.PathLike = ".",
             tag = uuid.uuid4().hex
             tmps = [f"{f}.{tag}.tmp" for _, f in todo]
             try:
-                msa_fn([q for q, _ in todo], tmps, **opts, **extra)
+                msa_fn([q for q, _ in todo], tmps, **opts, **kw)
                 for t, (_, f) in zip(tmps, todo):
                     os.replace(t, f)
             finally:
@@ -185,13 +196,30 @@ def create_app(msa_fn: Callable | None = None, root: str | os.PathLike = ".",
                                     fft_rank=rank, top_fft=top_fft, top_sw=top_sw,
                                     top_sw_affine=top_sw_affine, num_msas=num, bs=100_000_000)
 
-        parts = []
-        for i, path in enumerate(out_paths):
-            text = path.read_text().replace("@", "X")
-            rows = text.split("\n")
-            rows[0] = heads[i]
-            parts.append("\n".join(rows).encode())
-        content = b"\x00\n".join(parts)
+        def joined(paths: list[Path]) -> bytes:
+            parts = []
+            for i, path in enumerate(paths):
+                text = path.read_text().replace("@", "X")
+                rows = text.split("\n")
+                rows[0] = heads[i]
+                parts.append("\n".join(rows).encode())
+            return b"\x00\n".join(parts)
+
+        content = joined(out_paths)
+        env_content = b""
+        if env_msa_kwargs is not None:
+            env_folder = root / "cache_msa" / (
+                f"env_{chunks}_{num}_{top_fft}_{rank}_{top_sw}_{top_sw_affine}")
+            env_folder.mkdir(parents=True, exist_ok=True)
+            env_paths = [env_folder / f"{_sha(p)}.a3m" for p in proteins]
+            env_todo = [p for p in distinct if not (env_folder / f"{_sha(p)}.a3m").exists()]
+            if env_todo:
+                await run_in_threadpool(search, env_todo,
+                                        [str(env_folder / f"{_sha(p)}.a3m") for p in env_todo],
+                                        dict(env_msa_kwargs), fft_rank=rank, top_fft=top_fft,
+                                        top_sw=top_sw, top_sw_affine=top_sw_affine, num_msas=num,
+                                        bs=100_000_000)
+            env_content = joined(env_paths)
 
         tar_id = _sha(distinct[-1]) if distinct else _sha("")
         tar_dir = root / f"{tar_id}_tar"
@@ -204,8 +232,8 @@ def create_app(msa_fn: Callable | None = None, root: str | os.PathLike = ".",
             ti.size = len(content)
             tar.addfile(ti, io.BytesIO(content))
             ti2 = tarfile.TarInfo(name=ENV_A3M)
-            ti2.size = 0
-            tar.addfile(ti2, io.BytesIO(b""))
+            ti2.size = len(env_content)
+            tar.addfile(ti2, io.BytesIO(env_content))
         os.replace(tmp, tar_file)
         result_store[tar_id] = tar_file
         return {"status": "COMPLETE", "id": tar_id}
@@ -263,8 +291,12 @@ def main(argv=None) -> None:
     ap.add_argument("-port", "-p", default=8000, type=int, help="Port for server. ")
     args, _ = ap.parse_known_args(argv)
     import uvicorn
-    uvicorn.run(create_app(), host="0.0.0.0", port=args.port, reload=False, access_log=False,
-                workers=1)
+    env_kwargs = None
+    if os.environ.get("ENVDB"):  # an environmental database (.xbit), searched like DBPATH
+        from .db import SeqDB, env_chunks
+        env_kwargs = {"db": SeqDB.from_xbit(os.environ["ENVDB"], env_chunks())}
+    uvicorn.run(create_app(env_msa_kwargs=env_kwargs), host="0.0.0.0", port=args.port,
+                reload=False, access_log=False, workers=1)
 
 
 if __name__ == "__main__":

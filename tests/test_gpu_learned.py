@@ -1,0 +1,176 @@
+"""The learned prefilter's HIP kernels against a plain PyTorch fp32 reference
+of the same operations (on the CPU), and the proposal against its
+definition.
+
+* forward: embeddings of sequences of mixed lengths (with non-residue
+  letters) equal the reference that rounds weights and stored activations
+  to bf16 where the kernels do, within 2e-3 per component of the unit
+  vectors (fp32 accumulation in another order, tanh of another library);
+* backward: the gradients of a pair loss with respect to every parameter
+  block equal autograd's of the fp32 model within the error that bf16
+  operands bring (relative L2 error below 5e-2, cosine above 0.998);
+* the training step moves the loss down;
+* the FP8 table holds e4m3 of 16 * e, and rows built in chunks equal the
+  same sequences embedded in one batch, bit for bit;
+* proposal scores equal the fp32 dot products of the fp8 operands within
+  fp16 rounding, and the proposed candidates are the exact top-k of those
+  scores; the learned cascade's SW scores are the oracle's.
+"""
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+from conftest import announce
+
+pytestmark = pytest.mark.gpu
+
+AA = "ARNDCQEGHILKMFPSTWYV"
+
+
+def _bf(t):
+    return t.to(torch.bfloat16).to(torch.float32)
+
+
+def ref_forward(params, seqs, round_bf16=True):
+    """Encoder in fp32 on the CPU; round_bf16 rounds the weights, the
+    embedding rows and every stored activation to bf16 like the kernels."""
+    from sseqs_amd.learned import codes_of
+    r = _bf if round_bf16 else (lambda t: t)
+    E = params["E"]
+    out = []
+    for s in seqs:
+        c = torch.from_numpy(codes_of(s).astype(np.int64))
+        x = r(E[c]).T.unsqueeze(0)  # (1, 64, L)
+        for wn, bn in (("W1", "b1"), ("W2", "b2"), ("W3", "b3")):
+            W = r(params[wn]).permute(2, 1, 0)  # (Cout, Cin, 9)
+            u = F.conv1d(x, W, padding=4) + params[bn][None, :, None]
+            x = r(F.gelu(u, approximate="tanh"))
+        z = x[0].mean(dim=1)
+        n = z.norm()
+        out.append(z / n if n > 0 else torch.zeros_like(z))
+    return torch.stack(out)
+
+
+def _seqs(rng, n, lo=1, hi=300):
+    out = []
+    for _ in range(n):
+        L = int(rng.integers(lo, hi + 1))
+        s = "".join(rng.choice(list(AA + "XB"), L, p=[0.048] * 20 + [0.02, 0.02]))
+        out.append(s)
+    return out
+
+
+@pytest.fixture(scope="module")
+def enc(dev):
+    from sseqs_amd.learned import Encoder
+    return Encoder(dev, seed=3)
+
+
+def _params_cpu(enc):
+    return {k: torch.from_numpy(v.copy()) for k, v in enc.state().items()}
+
+
+def test_encoder_forward_matches_reference(enc):
+    announce("encoder forward vs fp32 reference")
+    rng = np.random.default_rng(1)
+    seqs = _seqs(rng, 40) + ["W", "XXXX", "ACDEFGHIKLMNPQRSTVWY" * 20]
+    got = enc.embed_strings(seqs).cpu()
+    want = ref_forward(_params_cpu(enc), seqs)
+    err = (got - want).abs().max().item()
+    assert err < 2e-3, err
+    # unit vectors, except the sequence without residues (zero embedding)
+    has = torch.tensor([any(c in AA for c in s) for s in seqs])
+    assert torch.allclose(got.norm(dim=1)[has], torch.ones(int(has.sum())), atol=1e-4)
+    assert got[~has].abs().max().item() == 0.0
+
+
+def test_encoder_backward_matches_autograd(enc, dev):
+    from sseqs_amd._lib import call, ptr, stream_of
+    from sseqs_amd.learned import Encoder
+    announce("encoder backward vs autograd")
+    rng = np.random.default_rng(2)
+    seqs = _seqs(rng, 24, 20, 250)
+    nq = 6
+    qi = np.repeat(np.arange(nq), 3).astype(np.int32)
+    ti = (nq + np.arange(18)).astype(np.int32)
+    y = rng.uniform(0.0, 0.9, 18).astype(np.float32)
+    e2 = Encoder(dev, weights=enc.state())
+    code, mask, rows = e2.rows_of_strings(seqs)
+    e, znorm, buf = e2.forward(code, mask, rows, train=True)
+    de = torch.zeros_like(e)
+    loss = torch.zeros(1, dtype=torch.float32, device=dev)
+    # device copies held in names until the launch has been enqueued
+    qi_d, ti_d, y_d = (torch.from_numpy(x).to(dev) for x in (qi, ti, y))
+    call("sseqs_enc_pair_loss", ptr(e), int(e.shape[0]), ptr(qi_d), ptr(ti_d), ptr(y_d), 18,
+         ptr(de), None, ptr(loss), stream_of(dev))
+    e2.backward(de, e, znorm, code, mask, rows, buf)
+    torch.cuda.synchronize()
+    params = {k: v.clone().requires_grad_(True) for k, v in _params_cpu(enc).items()}
+    emb = ref_forward(params, seqs, round_bf16=False)
+    cos = (emb[torch.from_numpy(qi).long()] * emb[torch.from_numpy(ti).long()]).sum(1)
+    ref_loss = ((cos - torch.from_numpy(y)) ** 2).mean()
+    ref_loss.backward()
+    assert abs(loss.item() - ref_loss.item()) < 2e-3 * max(1.0, ref_loss.item())
+    for name in ("E", "W1", "b1", "W2", "b2", "W3", "b3"):
+        g = e2.view(e2.g, name).cpu().flatten()
+        w = params[name].grad.flatten()
+        if name == "E":
+            g, w = g[: 20 * 64], w[: 20 * 64]
+        rel = ((g - w).norm() / w.norm()).item()
+        cosg = (g @ w / (g.norm() * w.norm())).item()
+        assert rel < 5e-2 and cosg > 0.998, (name, rel, cosg)
+
+
+def test_training_lowers_the_loss(dev):
+    from sseqs_amd.learned import Encoder, make_training_set, train
+    announce("encoder training on synthetic pairs")
+    ts = make_training_set(60_000, 64, 64, seed=5, device=dev, lo=50, hi=300)
+    assert ts.y.min() >= 0.0 and ts.y.max() <= 1.0
+    hom = ts.y > 0.3
+    assert hom.any() and (~hom).any()
+    enc = Encoder(dev, seed=4)
+    losses = train(enc, ts, steps=120, queries_per_step=16, pairs_per_query=16, lr=2e-3)
+    assert np.mean(losses[-2:]) < 0.7 * losses[0], losses
+
+
+def test_table_and_proposal(orc, enc, dev):
+    from sseqs_amd import synth
+    from sseqs_amd._lib import call, ptr, stream_of
+    from sseqs_amd.db import SeqDB
+    from sseqs_amd.learned import FP8_SCALE, EmbeddingTable, Proposer
+    from sseqs_amd.multi import MultiSearch
+    announce("fp8 table and MFMA proposal")
+    queries = [synth.make_query(L, 90 + i, seed=31) for i, L in enumerate((80, 230, 510))]
+    db = SeqDB.synthetic(12000, queries, seed=32, device=dev)
+    tab = EmbeddingTable(enc, db, rows=400_000)  # several chunks
+    ids = np.arange(0, db.n, 37)
+    seqs = db.strings(torch.from_numpy(ids).to(dev), 2048)
+    e_one = enc.embed_strings(seqs)
+    q8 = torch.empty((ids.size, 256), dtype=torch.uint8, device=dev)
+    call("sseqs_enc_to_fp8", ptr(e_one), e_one.numel(), FP8_SCALE, ptr(q8), stream_of(dev))
+    assert torch.equal(tab.table[torch.from_numpy(ids).to(dev)], q8)
+    # e4m3 (OCP) of 16 * e
+    dec = tab.table[:500].cpu().view(torch.float8_e4m3fn).float()
+    want = (FP8_SCALE * enc.embed_strings(db.strings(torch.arange(500, device=dev), 2048))).cpu()
+    assert torch.allclose(dec, want.to(torch.float8_e4m3fn).float(), atol=0, rtol=0)
+    # proposal scores and candidates
+    ms = MultiSearch(db, queries, top_prefilter=600, top_hits=50, proposer=Proposer(tab, group=16))
+    ms.run()
+    sc = ms.proposer._scores[:3].float().cpu()
+    qe = ms.proposer.table.enc.embed_strings(queries)
+    qq = (FP8_SCALE * qe).cpu().to(torch.float8_e4m3fn).float()
+    tt = tab.table.cpu().view(torch.float8_e4m3fn).float()
+    ref = qq @ tt.T
+    assert torch.allclose(sc, ref.half().float(), rtol=2e-3, atol=2e-3)
+    for i in range(3):
+        cand = ms.cand[i * 600:(i + 1) * 600].cpu().numpy()
+        top = orc.topk_desc(sc[i].numpy().astype(np.float64), 600)
+        assert np.array_equal(cand, np.sort(top)), i
+    # exact SW of the proposed candidates
+    ascii = orc.xbit_unpack(db.packed.cpu().numpy(), db.n_syms)
+    off, ln = db.seq_off.cpu().numpy(), db.seq_len.cpu().numpy()
+    for i, q in enumerate(queries):
+        cand = ms.cand[i * 600:(i + 1) * 600].cpu().numpy()
+        sw = orc.sw_affine_segments(orc.encode_query_sw(q), ascii, off[cand], ln[cand])
+        assert np.array_equal(ms.cs[i * 600:(i + 1) * 600].cpu().numpy(), sw)

@@ -56,13 +56,16 @@ def test_engine_alone_equals_mix(sworld, tmp_path):
 
 
 def test_engine_hbm_budget_splits_sweeps(sworld):
-    """A budget that holds one query's buffers (plus the sweep's own) admits
-    one query per sweep."""
+    """A budget that no two consecutive queries' buffers (plus their sweep's
+    own) fit admits one query per sweep (the first query of a sweep is
+    always admitted)."""
     from sseqs_amd.serve import SearchEngine, query_bytes, sweep_bytes
     db, queries = sworld["db"], sworld["queries"]
     announce("serving engine: HBM budget admission")
-    lq = max(len(q) for q in queries)
-    budget = sweep_bytes(db, lq, 200) + query_bytes(db, lq, 2000, 200) + 1
+    ls = [len(q) for q in queries[:5]]
+    pair = [sweep_bytes(db, max(a, b), 200) + query_bytes(db, a, 2000, 200) +
+            query_bytes(db, b, 2000, 200) for a, b in zip(ls, ls[1:])]
+    budget = min(pair) - 1
     with SearchEngine(db, top_prefilter=2000, top_hits=200, max_batch=8,
                       hbm_budget=budget) as eng:
         futs = [eng.submit(q) for q in queries[:5]]

@@ -185,3 +185,27 @@ def test_concurrent_tickets_search_once(tmp_path, monkeypatch):
     for i in range(2):
         text = out[i]["uniref.a3m"].decode()
         assert text in (f">r{j}\nMKTAYIAK\n>hit\nMKTAYIAK\n" for j in range(2))
+
+
+def test_environmental_database(tmp_path, monkeypatch):
+    """With env_msa_kwargs the env member holds a second search's a3m (the
+    reference's is always empty); both searches are cached."""
+    for k in ("MSAS_NUM", "MSAS_FFT_RANK", "MSAS_TOP_FFT", "MSAS_TOP_SW", "MSAS_TOP_SW_AFFINE"):
+        monkeypatch.delenv(k, raising=False)
+    calls = []
+
+    def stub(queries, filenames, db=None, **opts):
+        calls.append(db)
+        for q, fn in zip(queries, filenames):
+            with open(fn, "w") as fh:
+                fh.write(f">original_query\n{q}\n>{db}_hit\n{q}\n")
+
+    client = TestClient(create_app(msa_fn=stub, root=tmp_path, chunks=4,
+                                   msa_kwargs={"db": "UNIREF"}, env_msa_kwargs={"db": "ENV"}))
+    r = client.post("/ticket/msa", data={"q": ">a\nMKTAYIAK"})
+    members = untar(client.get(f"/result/download/{r.json()['id']}").content)
+    assert members["uniref.a3m"].decode() == ">a\nMKTAYIAK\n>UNIREF_hit\nMKTAYIAK\n"
+    assert members[ENV_A3M].decode() == ">a\nMKTAYIAK\n>ENV_hit\nMKTAYIAK\n"
+    assert calls == ["UNIREF", "ENV"]
+    client.post("/ticket/msa", data={"q": ">b\nMKTAYIAK"})
+    assert calls == ["UNIREF", "ENV"]  # both cached
