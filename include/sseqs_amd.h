@@ -200,6 +200,19 @@ size_t sseqs_topk_ws_bytes(int64_t n);
 int sseqs_topk(const void* scores, int dtype, int64_t n, int64_t k, int sorted, int64_t* out_idx,
                void* ws, size_t ws_bytes, hipStream_t stream);
 
+/* sseqs_topk without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It has no counterpart in the
+ * reference.  Same arguments, order and results as sseqs_topk; the
+ * workspace is a library-owned static device array of 512 KiB per GPU
+ * (held until the process ends), so n is limited to
+ * sseqs_topk_noalloc_max_n() (393 M scores) and larger n is rejected with
+ * an error.  It calls no hipMalloc/hipFree.  Calls on one GPU share that
+ * array: calls that overlap on one GPU (other streams or threads) race. */
+int64_t sseqs_topk_noalloc_max_n(void);
+int sseqs_topk_noalloc(const void* scores, int dtype, int64_t n, int64_t k, int sorted,
+                       int64_t* out_idx, hipStream_t stream);
+
 /* sseqs_topk over many segments in one launch: the per-batch selects of the
  * reference cascade (sseqs/msa.py:297-310, 324, 335) for all batches of a
  * sweep.  Segment j holds scores[seg_begin[j] .. + seg_n[j]) and selects its

@@ -73,6 +73,8 @@ SIGNATURES = {
     "sseqs_propose": (_i32, [_vp, _i64, _vp, _i32, _vp, _i64, _vp]),
     "sseqs_topk_ws_bytes": (_sz, [_i64]),
     "sseqs_topk": (_i32, [_vp, _i32, _i64, _i64, _i32, _vp, _vp, _sz, _vp]),
+    "sseqs_topk_noalloc_max_n": (_i64, []),
+    "sseqs_topk_noalloc": (_i32, [_vp, _i32, _i64, _i64, _i32, _vp, _vp]),
     "sseqs_topk_segments": (_i32, [_vp, _i32, _vp, _vp, _vp, _vp, _i64, _i32, _i32, _i32, _i32,
                                    _i32, _vp, _vp]),
     "sseqs_backtrack_ws_bytes": (_sz, [_i32, _i64, _i32]),

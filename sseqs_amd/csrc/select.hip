@@ -8,6 +8,9 @@
 // ones with key == T, in ascending index order.  The optional sort orders
 // them by (key desc, index asc) with a bitonic sort in LDS.  The tie rule
 // (smaller index first) is the build's definition; torch.topk leaves it open.
+#include <mutex>
+#include <string>
+
 #include "common.h"
 
 namespace sseqs {
@@ -541,6 +544,66 @@ extern "C" int sseqs_topk(const void* scores, int dtype, int64_t n, int64_t k, i
         if (int r = check_launch("topk sort")) return r;
     }
     return 0;
+}
+
+// ------------------------------------------------ sseqs_topk_noalloc --
+//
+// sseqs_topk on a library-owned scratch array instead of a caller
+// workspace, for callers that cannot allocate.  `scratch` lives in device
+// memory of every GPU the library runs on (one copy per device, held until
+// the process ends).  Host code never passes the array's name to a kernel:
+// the name is a host-side stand-in, so each device's address is looked up
+// once with hipGetSymbolAddress (with that device current) and kept in
+// g_scratch_ptr.  Every call on one device shares its copy; k_sel_init
+// clears the histogram and state at the start of each call (a kernel, not
+// hipMemsetAsync), and the count pass writes every per-tile counter, so
+// nothing depends on the previous call's contents.  Calls that overlap on
+// one device (two streams, or two threads) race on the array.
+
+constexpr size_t NOALLOC_SCRATCH_BYTES = 512 * 1024;
+static __device__ __align__(256) uint8_t scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Largest n whose workspace fits the scratch array.
+int64_t noalloc_max_n() {
+    const size_t fixed = (size_t)SEL_BINS * 4 + 64 + (size_t)SEL_SORT_MAX * 8 + 256;
+    return (int64_t)((NOALLOC_SCRATCH_BYTES - fixed) / 16) * SEL_TILE;
+}
+
+// Device address of this device's scratch copy (looked up on first use).
+int scratch_ptr(void** out) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(-4, "topk_noalloc: hipGetDevice failed");
+    if (dev < 0 || dev >= kMaxDevices) return fail(-4, "topk_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail((int)(e != hipSuccess ? e : hipErrorInvalidSymbol),
+                        std::string("topk_noalloc: hipGetSymbolAddress failed: ") +
+                            hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+}  // namespace
+
+extern "C" int64_t sseqs_topk_noalloc_max_n(void) { return noalloc_max_n(); }
+
+extern "C" int sseqs_topk_noalloc(const void* scores, int dtype, int64_t n, int64_t k, int sorted,
+                                  int64_t* out_idx, hipStream_t stream) {
+    if (n > noalloc_max_n())
+        return fail(-1, "topk_noalloc: n = " + std::to_string(n) + " exceeds the scratch limit of " +
+                            std::to_string(noalloc_max_n()) + " scores");
+    void* ws = nullptr;
+    if (int r = scratch_ptr(&ws)) return r;
+    return sseqs_topk(scores, dtype, n, k, sorted, out_idx, ws, NOALLOC_SCRATCH_BYTES, stream);
 }
 
 extern "C" int sseqs_topk_segments(const void* scores, int dtype, const int64_t* seg_begin,
