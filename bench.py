@@ -572,6 +572,8 @@ def main() -> int:
                "sw_tuning": {k: v for k, v in ms.tuning.get("sw", {}).items()},
                "sw_scores_equal_single_query_runs": same,
                "candidate_residues": cand_res, "setup_s": round(setup, 1)}
+        if proposer is not None:
+            res["query_embedding_ms"] = round(1e3 * ms.query_embed_s, 1)
         log(json.dumps(res))
         return res, ms
 
@@ -630,9 +632,21 @@ def main() -> int:
                                                                device=dev)).item()) + 1
                 del cum
                 r[f"{name}_top_prefilter_for_0.99"] = K99
-            # the plain cascade at its 0.99 operating point, timed (for the
+            # each cascade at its own 0.99 operating point, timed (for the
             # first --recall-timed queries of the sample)
             if len(rec) < args.recall_timed:
+                from sseqs_amd.multi import MultiSearch
+                KL = r["learned_top_prefilter_for_0.99"]
+                m1 = MultiSearch(db, [qtext], KL, k, proposer=prop, backtrack=True)
+                m1.run()
+                torch.cuda.synchronize()
+                t = time.perf_counter()
+                m1.run()
+                torch.cuda.synchronize()
+                r["learned_ms_at_0.99"] = round(1e3 * (time.perf_counter() - t), 1)
+                r["learned_recall_measured_at_0.99"] = round(
+                    min(k, int((m1.hit_sc >= kth).sum().item())) / k, 4)
+                del m1
                 K99 = r["plain_top_prefilter_for_0.99"]
                 global_search(db, q, K99, k, scores_buf=scores_buf)
                 torch.cuda.synchronize()
@@ -651,6 +665,10 @@ def main() -> int:
                "plain_ms_per_query_at_0.99": round(float(np.mean([r["plain_ms_at_0.99"]
                                                                   for r in rec
                                                                   if "plain_ms_at_0.99" in r])), 1),
+               "learned_ms_per_query_at_0.99": round(float(np.mean([r["learned_ms_at_0.99"]
+                                                                    for r in rec
+                                                                    if "learned_ms_at_0.99" in r])),
+                                                     1),
                "recall_queries": len(rec),
                "learned_recall_min": min(r["learned_recall_at_top_prefilter"] for r in rec),
                "learned_recall_mean": round(float(np.mean(
@@ -663,21 +681,22 @@ def main() -> int:
         if plain_batched:
             out["plain_batched_ms_per_query_top_prefilter"] = plain_batched["ms_per_query"]
         out["speedup_vs_plain_at_0.99"] = round(out["plain_ms_per_query_at_0.99"] /
-                                                out["learned_ms_per_query"], 1)
+                                                out["learned_ms_per_query_at_0.99"], 1)
         log(json.dumps({k_: v for k_, v in out.items() if k_ not in ("sweep", "recall_sample")}))
         del ms
         torch.cuda.empty_cache()
         if args.serve_requests > 0:
-            # serving at each prefilter's 0.99-recall operating point
-            k_l = max(args.top_prefilter, out["learned_k99_max"])
-            k_p = max(r["plain_top_prefilter_for_0.99"] for r in rec)
+            # serving: the learned prefilter at the configured candidate count
+            # and both prefilters at their median 0.99-recall operating point
+            k_l = int(np.median([r["learned_top_prefilter_for_0.99"] for r in rec]))
+            k_p = int(np.median([r["plain_top_prefilter_for_0.99"] for r in rec]))
             out["serving"] = [
                 side("serving learned", lambda: measure_serving(
-                    args.serve_rate_high, args.serve_requests, k_l, prop, "learned")),
-                side("serving learned", lambda: measure_serving(
-                    args.serve_rate_low, max(16, args.serve_requests // 10), k_l, prop,
+                    args.serve_rate_high, args.serve_requests, args.top_prefilter, prop,
                     "learned")),
-                side("serving plain", lambda: measure_serving(
+                side("serving learned 0.99", lambda: measure_serving(
+                    args.serve_rate_low, args.serve_plain_requests, k_l, prop, "learned")),
+                side("serving plain 0.99", lambda: measure_serving(
                     args.serve_rate_low, args.serve_plain_requests, k_p, None, "plain")),
             ]
         del tab, prop
@@ -723,6 +742,15 @@ def main() -> int:
         log(json.dumps(res))
         return res
 
+    def side(name, fn):
+        """An auxiliary measurement: a failure is logged and reported as null
+        rather than taking the headline line with it."""
+        try:
+            return fn()
+        except Exception as e:  # noqa: BLE001 - reported, not hidden
+            log(f"{name} failed: {type(e).__name__}: {e}")
+            return None
+
     if args.batch_only:
         measure_batch(q200)
         return 0
@@ -734,15 +762,6 @@ def main() -> int:
         print(json.dumps({"batched": measure_batched(batch_queries[: args.batched_queries])[0]}),
               flush=True)
         return 0
-    def side(name, fn):
-        """An auxiliary measurement: a failure is logged and reported as null
-        rather than taking the headline line with it."""
-        try:
-            return fn()
-        except Exception as e:  # noqa: BLE001 - reported, not hidden
-            log(f"{name} failed: {type(e).__name__}: {e}")
-            return None
-
     r200, launches200 = measure(q200)
     rbatch = side("batch cascade", lambda: measure_batch(q200)) if args.batch_steps > 0 else None
     re2e = side("msa call", lambda: measure_msa(q200)) if args.batch_steps > 0 else None

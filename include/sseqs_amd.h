@@ -294,13 +294,13 @@ int sseqs_enc_pool(const void* h3, const int64_t* row0, const int32_t* len, int6
 int sseqs_enc_pool_grad(const float* de, const float* e, const float* znorm, const void* U3,
                         const int64_t* row0, const int32_t* len, int64_t nseq, void* dU3,
                         hipStream_t stream);
-/* Squared error of cos_j = e[qi[j]] . e[ti[j]] against y[j], averaged over
- * npair; its gradient is added to de, cos_j to cos_out (if not null), the
- * loss to *loss (if not null).  e holds nseq rows; a pair naming a row
- * outside 0..nseq-1 is skipped. */
+/* Squared error of cos_j = e[qi[j]] . e[ti[j]] against y[j], weighted by
+ * w[j] (1 when w is null) and averaged over npair; its gradient is added to
+ * de, cos_j to cos_out (if not null), the loss to *loss (if not null).  e
+ * holds nseq rows; a pair naming a row outside 0..nseq-1 is skipped. */
 int sseqs_enc_pair_loss(const float* e, int64_t nseq, const int32_t* qi, const int32_t* ti,
-                        const float* y, int64_t npair, float* de, float* cos_out, float* loss,
-                        hipStream_t stream);
+                        const float* y, const float* w, int64_t npair, float* de, float* cos_out,
+                        float* loss, hipStream_t stream);
 /* AdamW step `step` (from 1) over n fp32 parameters. */
 int sseqs_enc_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1,
                    float b2, float eps, float wd, int step, hipStream_t stream);

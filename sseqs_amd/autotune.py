@@ -27,7 +27,7 @@ import torch
 
 from .multi import SW_CLASSES, TUNING_PATH, MultiSearch
 
-ITEMS = (256, 512, 1024, 2048, 4096)
+ITEMS = (64, 128, 256, 512, 1024, 2048, 4096)
 BLOCKS_PER_CU = (1, 2, 4, 8)
 CORR_MT = (2, 4, 8)
 

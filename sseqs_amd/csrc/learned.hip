@@ -351,7 +351,8 @@ __global__ __launch_bounds__(256) void k_pool_grad(const float* __restrict__ de,
 __global__ __launch_bounds__(256) void k_pair_loss(const float* __restrict__ e, int64_t nseq,
                                                    const int32_t* __restrict__ qi,
                                                    const int32_t* __restrict__ ti,
-                                                   const float* __restrict__ y, int64_t npair,
+                                                   const float* __restrict__ y,
+                                                   const float* __restrict__ wt, int64_t npair,
                                                    float* __restrict__ de, float* __restrict__ cos_out,
                                                    float* __restrict__ loss) {
     const int lane = threadIdx.x & 63;
@@ -365,7 +366,8 @@ __global__ __launch_bounds__(256) void k_pair_loss(const float* __restrict__ e, 
 #pragma unroll
     for (int x = 0; x < 4; ++x) d += a[lane * 4 + x] * b[lane * 4 + x];
     for (int off = 32; off > 0; off >>= 1) d += __shfl_xor(d, off, 64);
-    const float g = 2.f * (d - y[j]) / (float)npair;
+    const float wj = wt ? wt[j] : 1.f;
+    const float g = 2.f * wj * (d - y[j]) / (float)npair;
 #pragma unroll
     for (int x = 0; x < 4; ++x) {
         atomicAdd(&de[(int64_t)qi[j] * 256 + lane * 4 + x], g * b[lane * 4 + x]);
@@ -373,7 +375,7 @@ __global__ __launch_bounds__(256) void k_pair_loss(const float* __restrict__ e, 
     }
     if (lane == 0) {
         if (cos_out) cos_out[j] = d;
-        if (loss) atomicAdd(loss, (d - y[j]) * (d - y[j]) / (float)npair);
+        if (loss) atomicAdd(loss, wj * (d - y[j]) * (d - y[j]) / (float)npair);
     }
 }
 
@@ -584,11 +586,12 @@ extern "C" int sseqs_enc_pool_grad(const float* de, const float* e, const float*
 }
 
 extern "C" int sseqs_enc_pair_loss(const float* e, int64_t nseq, const int32_t* qi,
-                                   const int32_t* ti, const float* y, int64_t npair, float* de,
-                                   float* cos_out, float* loss, hipStream_t stream) {
+                                   const int32_t* ti, const float* y, const float* w,
+                                   int64_t npair, float* de, float* cos_out, float* loss,
+                                   hipStream_t stream) {
     if (npair <= 0) return 0;
     hipLaunchKernelGGL(k_pair_loss, dim3((unsigned)ceil_div(npair, 4)), dim3(256), 0, stream, e,
-                       nseq, qi, ti, y, npair, de, cos_out, loss);
+                       nseq, qi, ti, y, w, npair, de, cos_out, loss);
     return check_launch("enc_pair_loss");
 }
 

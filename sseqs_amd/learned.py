@@ -278,6 +278,7 @@ class TrainingSet:
     q_ids: np.ndarray     # int32 [npair] query index
     t_ids: np.ndarray     # int64 [npair] database id of the target
     y: np.ndarray         # float32 [npair] SW(q, t) / SW(q, q)
+    homolog: np.ndarray   # bool [npair] the target is a planted homolog of the query
 
 
 def make_training_set(n_db: int, n_queries: int, randoms_per_query: int, seed: int = 11,
@@ -303,12 +304,14 @@ def make_training_set(n_db: int, n_queries: int, randoms_per_query: int, seed: i
     hq_sorted = hq[order]
     starts = np.searchsorted(hq_sorted, np.arange(n_queries))
     ends = np.searchsorted(hq_sorted, np.arange(n_queries), side="right")
-    cand, qidx = [], []
+    cand, qidx, ishom = [], [], []
     for i in range(n_queries):
         hom = order[starts[i]:ends[i]]
         rnd = rng.choice(bg, size=randoms_per_query, replace=False)
-        c = np.sort(np.concatenate([hom, rnd]))
-        cand.append(c)
+        c = np.concatenate([hom, rnd])
+        srt = np.argsort(c, kind="stable")
+        cand.append(c[srt])
+        ishom.append((np.arange(c.size) < hom.size)[srt])
         qidx.append(np.full(c.size, i, np.int32))
     # exact SW of every pair with the batched SW queue
     ms = MultiSearch(db, queries, top_prefilter=1, top_hits=1, backtrack=False, tuning={})
@@ -325,24 +328,41 @@ def make_training_set(n_db: int, n_queries: int, randoms_per_query: int, seed: i
     selfs = np.array([float(sw(q, [q])[0]) for q in queries], dtype=np.float32)
     q_all = np.concatenate(qidx)
     y = np.clip(scores / np.maximum(selfs[q_all], 1.0), 0.0, 1.0).astype(np.float32)
-    return TrainingSet(db, queries, q_all, c_all, y)
+    return TrainingSet(db, queries, q_all, c_all, y, np.concatenate(ishom))
 
 
 def train(enc: Encoder, ts: TrainingSet, steps: int, queries_per_step: int = 32,
           pairs_per_query: int = 32, lr: float = 1e-3, seed: int = 0, log=None,
-          time_budget_s: float | None = None) -> list[float]:
-    """AdamW on the squared error between cosine and normalised SW score.
-    Each step takes queries_per_step queries and pairs_per_query of each
-    query's pairs (half homologs, half background where available)."""
+          time_budget_s: float | None = None, homolog_weight: float = 4.0,
+          lr_floor: float = 0.05) -> list[float]:
+    """AdamW on the squared error between cosine and normalised SW score,
+    homolog pairs weighted ``homolog_weight`` (the proposal must rank the
+    homologs of a query among themselves, background pairs only need to
+    stay low).  Each step takes queries_per_step queries and pairs_per_query
+    of each query's pairs, half homologs and half background where
+    available.  The learning rate warms up over 200 steps, then follows a
+    cosine to lr_floor * lr at the last step."""
     rng = np.random.default_rng(seed)
     nq = len(ts.queries)
-    by_q = [np.nonzero(ts.q_ids == i)[0] for i in range(nq)]
+    # pairs are stored query by query: each query's range, split by kind
+    lo = np.searchsorted(ts.q_ids, np.arange(nq))
+    hi = np.searchsorted(ts.q_ids, np.arange(nq), side="right")
+    by_q_h = [lo[i] + np.nonzero(ts.homolog[lo[i]:hi[i]])[0] for i in range(nq)]
+    by_q_b = [lo[i] + np.nonzero(~ts.homolog[lo[i]:hi[i]])[0] for i in range(nq)]
+    w_all = np.where(ts.homolog, homolog_weight, 1.0).astype(np.float32)
     losses = []
     t0 = time.time()
+    half = pairs_per_query // 2
+
+    def pick(i):
+        h = rng.choice(by_q_h[i], size=min(half, by_q_h[i].size), replace=False)
+        nb = pairs_per_query - h.size
+        b = rng.choice(by_q_b[i], size=min(nb, by_q_b[i].size), replace=False)
+        return np.concatenate([h, b])
+
     for step in range(steps):
         qs = rng.choice(nq, size=min(queries_per_step, nq), replace=False)
-        pair_idx = np.concatenate([rng.choice(by_q[i], size=min(pairs_per_query, by_q[i].size),
-                                              replace=False) for i in qs])
+        pair_idx = np.concatenate([pick(i) for i in qs])
         tids = ts.t_ids[pair_idx]
         # sequences of the step: the queries, then the targets
         qcode, qmask, qrows = enc.rows_of_strings([ts.queries[i] for i in qs])
@@ -358,13 +378,15 @@ def train(enc: Encoder, ts: TrainingSet, steps: int, queries_per_step: int = 32,
         qi = torch.from_numpy(np.array([pos[q] for q in ts.q_ids[pair_idx]], np.int32)).to(enc.dev)
         ti = torch.arange(len(qs), len(qs) + tids.size, dtype=torch.int32, device=enc.dev)
         y = torch.from_numpy(ts.y[pair_idx]).to(enc.dev)
+        wt = torch.from_numpy(w_all[pair_idx]).to(enc.dev)
         de = torch.zeros_like(e)
         loss = torch.zeros(1, dtype=torch.float32, device=enc.dev)
-        call("sseqs_enc_pair_loss", ptr(e), int(e.shape[0]), ptr(qi), ptr(ti), ptr(y), tids.size,
-             ptr(de), None, ptr(loss), stream_of(enc.dev))
+        call("sseqs_enc_pair_loss", ptr(e), int(e.shape[0]), ptr(qi), ptr(ti), ptr(y), ptr(wt),
+             tids.size, ptr(de), None, ptr(loss), stream_of(enc.dev))
         enc.backward(de, e, znorm, code, mask, rows, buf)
         warm = min(1.0, (step + 1) / 200)
-        enc.adam_step(lr * warm)
+        cos = lr_floor + (1 - lr_floor) * 0.5 * (1 + math.cos(math.pi * step / max(1, steps - 1)))
+        enc.adam_step(lr * warm * cos)
         if step % 50 == 0 or step == steps - 1:
             losses.append(float(loss.item()))
             if log:
@@ -462,6 +484,29 @@ class Proposer:
         call("sseqs_propose", ptr(tab.table), tab.n, ptr(q8), Gp, ptr(out), tab.n, st)
         return out[: len(queries)]
 
+    def prepare(self, ms) -> None:
+        """Set-up of a sweep (before any capture): the queries' fp8
+        embedding rows, in groups of at most ``group`` (each padded to a
+        multiple of 16 with zero rows)."""
+        tab = self.table
+        st = stream_of(ms.dev)
+        t0 = time.perf_counter()
+        e = tab.enc.embed_strings(ms.texts)
+        groups = []
+        for g0 in range(0, ms.nq, self.group):
+            g1 = min(ms.nq, g0 + self.group)
+            Gp = (g1 - g0 + 15) // 16 * 16
+            q = torch.zeros((Gp, DIM), dtype=torch.float32, device=ms.dev)
+            q[: g1 - g0] = e[g0:g1]
+            q8 = torch.empty((Gp, DIM), dtype=torch.uint8, device=ms.dev)
+            call("sseqs_enc_to_fp8", ptr(q), q.numel(), FP8_SCALE, ptr(q8), st)
+            groups.append((g0, g1, q8))
+        ms.learned_groups = groups
+        if self._scores is None or self._scores.shape[1] != tab.n:
+            self._scores = torch.empty((self.group, tab.n), dtype=torch.float16, device=ms.dev)
+        torch.cuda.synchronize(ms.dev)
+        ms.query_embed_s = time.perf_counter() - t0
+
     def select(self, ms) -> None:
         if not self.use_learned:
             saved = ms.proposer
@@ -472,26 +517,17 @@ class Proposer:
                 ms.proposer = saved
             return
         tab = self.table
-        enc = tab.enc
         st = stream_of(ms.dev)
-        e = enc.embed_strings(ms.texts)
-        G = self.group
-        if self._scores is None or self._scores.shape[1] != tab.n:
-            self._scores = torch.empty((G, tab.n), dtype=torch.float16, device=ms.dev)
-        sel_ws = ms.sel_ws
-        for g0 in range(0, ms.nq, G):
-            g1 = min(ms.nq, g0 + G)
-            Gp = (g1 - g0 + 15) // 16 * 16
-            q = torch.zeros((Gp, DIM), dtype=torch.float32, device=ms.dev)
-            q[: g1 - g0] = e[g0:g1]
-            q8 = torch.empty((Gp, DIM), dtype=torch.uint8, device=ms.dev)
-            call("sseqs_enc_to_fp8", ptr(q), q.numel(), FP8_SCALE, ptr(q8), st)
-            call("sseqs_propose", ptr(tab.table), tab.n, ptr(q8), Gp, ptr(self._scores), tab.n, st)
+        if getattr(ms, "learned_groups", None) is None:
+            self.prepare(ms)
+        for g0, g1, q8 in ms.learned_groups:
+            call("sseqs_propose", ptr(tab.table), tab.n, ptr(q8), q8.shape[0], ptr(self._scores),
+                 tab.n, st)
             for j in range(g1 - g0):
                 i = g0 + j
                 call("sseqs_topk", self._scores.data_ptr() + 2 * j * tab.n, _lib.F16, tab.n,
-                     ms.kc, 0, ms.cand.data_ptr() + 8 * i * ms.kc, ptr(sel_ws), sel_ws.numel(),
-                     st)
+                     ms.kc, 0, ms.cand.data_ptr() + 8 * i * ms.kc, ptr(ms.sel_ws),
+                     ms.sel_ws.numel(), st)
 
 
 def exhaustive_top(db, query: str, k: int, chunk: int = 25_000_000) -> tuple[int, torch.Tensor]:

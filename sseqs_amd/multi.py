@@ -155,6 +155,9 @@ class MultiSearch:
             self.info = torch.zeros((self.nq, self.kh, 6), dtype=torch.int32, device=dev)
         for q in self.qs:  # length ranges are computed (with a host read) once
             db.length_ranges(q.length)
+        self.learned_groups = None
+        if proposer is not None and hasattr(proposer, "prepare"):
+            proposer.prepare(self)  # query-side work with host copies, before any capture
         self.graph: torch.cuda.CUDAGraph | None = None
 
     def set_sw_tuning(self, tuning: dict) -> None:

@@ -1,7 +1,7 @@
 """Train the learned prefilter's encoder and save its weights.
 
 ``python -m sseqs_amd.train_encoder [--pairs-db 5000000 --queries 5000
---randoms 1500 --steps 20000 --time-budget 240 --out PATH]`` builds the
+--randoms 1500 --steps 50000 --time-budget 300 --out PATH]`` builds the
 synthetic training set (every second sequence of a --pairs-db database is a
 planted homolog of one of --queries queries with lengths uniform in
 [50, 1000]; each query is paired with all its homologs and --randoms
@@ -27,11 +27,12 @@ def main(argv=None) -> int:
     ap.add_argument("--pairs-db", type=int, default=5_000_000)
     ap.add_argument("--queries", type=int, default=5000)
     ap.add_argument("--randoms", type=int, default=1500)
-    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--steps", type=int, default=50000)
     ap.add_argument("--queries-per-step", type=int, default=32)
     ap.add_argument("--pairs-per-query", type=int, default=32)
     ap.add_argument("--lr", type=float, default=1e-3)
-    ap.add_argument("--time-budget", type=float, default=240.0)
+    ap.add_argument("--time-budget", type=float, default=300.0)
+    ap.add_argument("--homolog-weight", type=float, default=4.0)
     ap.add_argument("--seed", type=int, default=11)
     ap.add_argument("--out", default=str(WEIGHTS_PATH))
     args = ap.parse_args(argv)
@@ -44,7 +45,8 @@ def main(argv=None) -> int:
     enc = Encoder(dev, seed=args.seed)
     t1 = time.time()
     losses = train(enc, ts, args.steps, args.queries_per_step, args.pairs_per_query, args.lr,
-                   seed=args.seed, log=log, time_budget_s=args.time_budget)
+                   seed=args.seed, log=log, time_budget_s=args.time_budget,
+                   homolog_weight=args.homolog_weight)
     secs = time.time() - t1
     enc.save(args.out)
     print(json.dumps({"pairs": int(ts.y.size), "queries": len(ts.queries),

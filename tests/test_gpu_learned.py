@@ -102,8 +102,8 @@ def test_encoder_backward_matches_autograd(enc, dev):
     loss = torch.zeros(1, dtype=torch.float32, device=dev)
     # device copies held in names until the launch has been enqueued
     qi_d, ti_d, y_d = (torch.from_numpy(x).to(dev) for x in (qi, ti, y))
-    call("sseqs_enc_pair_loss", ptr(e), int(e.shape[0]), ptr(qi_d), ptr(ti_d), ptr(y_d), 18,
-         ptr(de), None, ptr(loss), stream_of(dev))
+    call("sseqs_enc_pair_loss", ptr(e), int(e.shape[0]), ptr(qi_d), ptr(ti_d), ptr(y_d), None,
+         18, ptr(de), None, ptr(loss), stream_of(dev))
     e2.backward(de, e, znorm, code, mask, rows, buf)
     torch.cuda.synchronize()
     params = {k: v.clone().requires_grad_(True) for k, v in _params_cpu(enc).items()}
