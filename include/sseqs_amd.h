@@ -301,6 +301,12 @@ int sseqs_enc_pool_grad(const float* de, const float* e, const float* znorm, con
 int sseqs_enc_pair_loss(const float* e, int64_t nseq, const int32_t* qi, const int32_t* ti,
                         const float* y, const float* w, int64_t npair, float* de, float* cos_out,
                         float* loss, hipStream_t stream);
+/* Soft-target InfoNCE: rows 0..nq-1 of e are queries, rows nq..nq+nt-1
+ * targets (nt <= 8192).  Loss -scale * sum_t P[q][t] log softmax_t(e_q . e_t
+ * / tau) summed over the queries whose P row is not all zero, added to
+ * *loss; its gradient is added to de.  G is an nq x nt float workspace. */
+int sseqs_enc_infonce(const float* e, int nq, int nt, const float* P, float tau, float scale,
+                      float* G, float* de, float* loss, hipStream_t stream);
 /* AdamW step `step` (from 1) over n fp32 parameters. */
 int sseqs_enc_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr, float b1,
                    float b2, float eps, float wd, int step, hipStream_t stream);

@@ -65,6 +65,8 @@ SIGNATURES = {
     "sseqs_enc_pool": (_i32, [_vp, _vp, _vp, _i64, _vp, _vp, _vp]),
     "sseqs_enc_pool_grad": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _vp]),
     "sseqs_enc_pair_loss": (_i32, [_vp, _i64, _vp, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp]),
+    "sseqs_enc_infonce": (_i32, [_vp, _i32, _i32, _vp, C.c_float, C.c_float, _vp, _vp, _vp,
+                                 _vp]),
     "sseqs_enc_adam": (_i32, [_vp, _vp, _vp, _vp, _i64, C.c_float, C.c_float, C.c_float, C.c_float,
                               C.c_float, _i32, _vp]),
     "sseqs_enc_pack_weights": (_i32, [_vp, _i32, _i32, _vp, _vp, _vp]),

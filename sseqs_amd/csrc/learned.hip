@@ -379,6 +379,94 @@ __global__ __launch_bounds__(256) void k_pair_loss(const float* __restrict__ e, 
     }
 }
 
+// Soft-target InfoNCE over a step's sequences: rows 0..nq-1 of e are the
+// queries, rows nq..nq+nt-1 the targets.  For query q the logits are
+// z_t = e_q . e_t / tau over every target and, with the target distribution
+// P[q][.] (a row summing to 1, or all zero to leave the query out), the
+// loss is -scale * sum_t P_qt log softmax(z)_t.  With
+// G_qt = scale * (softmax(z)_t - P_qt), de_q += sum_t G_qt e_t / tau here and
+// de_t += sum_q G_qt e_q / tau in k_infonce_targets.  One block per query;
+// the logits live in LDS (nt <= INFONCE_MAX_T).
+constexpr int INFONCE_MAX_T = 8192;
+
+__device__ inline float block_reduce_256(float v, float* red, bool is_max) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const float o = __shfl_xor(v, off, 64);
+        v = is_max ? fmaxf(v, o) : v + o;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return is_max ? fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]))
+                  : red[0] + red[1] + red[2] + red[3];
+}
+
+__global__ __launch_bounds__(256) void k_infonce(const float* __restrict__ e, int nq, int nt,
+                                                 const float* __restrict__ P, float inv_tau,
+                                                 float scale, float* __restrict__ G,
+                                                 float* __restrict__ de, float* __restrict__ loss) {
+    __shared__ float s_q[256];
+    __shared__ float s_z[INFONCE_MAX_T];
+    __shared__ float red[4];
+    const int q = blockIdx.x;
+    const int tid = threadIdx.x;
+    const float* Pq = P + (int64_t)q * nt;
+    float* Gq = G + (int64_t)q * nt;
+    s_q[tid] = e[(int64_t)q * 256 + tid];
+    __syncthreads();
+    float mx = -INFINITY, psum = 0.f;
+    for (int t = tid; t < nt; t += 256) {
+        const float4* r = reinterpret_cast<const float4*>(e + (int64_t)(nq + t) * 256);
+        float d = 0.f;
+#pragma unroll 8
+        for (int x = 0; x < 64; ++x) {
+            const float4 v = r[x];
+            d += v.x * s_q[4 * x] + v.y * s_q[4 * x + 1] + v.z * s_q[4 * x + 2] +
+                 v.w * s_q[4 * x + 3];
+        }
+        const float z = d * inv_tau;
+        s_z[t] = z;
+        mx = fmaxf(mx, z);
+        psum += Pq[t];
+    }
+    mx = block_reduce_256(mx, red, true);
+    psum = block_reduce_256(psum, red, false);
+    if (!(psum > 0.f)) {  // the query takes no part: zero gradient row
+        for (int t = tid; t < nt; t += 256) Gq[t] = 0.f;
+        return;
+    }
+    float se = 0.f;
+    for (int t = tid; t < nt; t += 256) se += __expf(s_z[t] - mx);
+    se = block_reduce_256(se, red, false);
+    const float lse = mx + __logf(se);
+    float lsum = 0.f;
+    for (int t = tid; t < nt; t += 256) {
+        const float pt = Pq[t];
+        const float z = s_z[t];
+        lsum -= pt * (z - lse);
+        const float g = scale * (__expf(z - lse) - pt);
+        Gq[t] = g;
+        s_z[t] = g;
+    }
+    lsum = block_reduce_256(lsum, red, false);
+    if (tid == 0) atomicAdd(loss, scale * lsum);
+    // de_q: thread tid owns channel tid (reads of row t are coalesced)
+    float acc = 0.f;
+    for (int t = 0; t < nt; ++t) acc += s_z[t] * e[(int64_t)(nq + t) * 256 + tid];
+    de[(int64_t)q * 256 + tid] += acc * inv_tau;
+}
+
+// de_t += sum_q G_qt e_q / tau: one block per target, one thread per channel.
+__global__ __launch_bounds__(256) void k_infonce_targets(const float* __restrict__ e, int nq,
+                                                         int nt, const float* __restrict__ G,
+                                                         float inv_tau, float* __restrict__ de) {
+    const int t = blockIdx.x;
+    const int c = threadIdx.x;
+    float acc = 0.f;
+    for (int q = 0; q < nq; ++q) acc += G[(int64_t)q * nt + t] * e[(int64_t)q * 256 + c];
+    de[(int64_t)(nq + t) * 256 + c] += acc * inv_tau;
+}
+
 // AdamW step on fp32 parameters.
 __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
                        float* __restrict__ v, int64_t n, float lr, float b1, float b2, float eps,
@@ -593,6 +681,20 @@ extern "C" int sseqs_enc_pair_loss(const float* e, int64_t nseq, const int32_t* 
     hipLaunchKernelGGL(k_pair_loss, dim3((unsigned)ceil_div(npair, 4)), dim3(256), 0, stream, e,
                        nseq, qi, ti, y, w, npair, de, cos_out, loss);
     return check_launch("enc_pair_loss");
+}
+
+extern "C" int sseqs_enc_infonce(const float* e, int nq, int nt, const float* P, float tau,
+                                 float scale, float* G, float* de, float* loss,
+                                 hipStream_t stream) {
+    if (nq <= 0 || nt <= 0) return fail(-1, "enc_infonce: need at least one query and one target");
+    if (nt > INFONCE_MAX_T) return fail(-1, "enc_infonce: more than 8192 targets");
+    if (!(tau > 0.f)) return fail(-1, "enc_infonce: tau must be positive");
+    hipLaunchKernelGGL(k_infonce, dim3(nq), dim3(256), 0, stream, e, nq, nt, P, 1.f / tau, scale,
+                       G, de, loss);
+    if (int r = check_launch("enc_infonce")) return r;
+    hipLaunchKernelGGL(k_infonce_targets, dim3(nt), dim3(256), 0, stream, e, nq, nt, G, 1.f / tau,
+                       de);
+    return check_launch("enc_infonce_targets");
 }
 
 extern "C" int sseqs_enc_adam(float* p, const float* g, float* m, float* v, int64_t n, float lr,

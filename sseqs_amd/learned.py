@@ -9,8 +9,11 @@ target, normalised SW score) pairs -- targets are planted homologs and
 background sequences of a synthetic database, the score is the affine SW
 score (reference stripe rule, 11/1) divided by the query's score against
 itself -- with hand-written forward and backward kernels (bf16 operands,
-fp32 accumulation) and AdamW, minimising the squared error between cosine
-and score.
+fp32 accumulation) and AdamW.  The loss is the squared error between cosine
+and score plus a soft-target InfoNCE term over the step's sequences (each
+query against every target of the step, its own pairs weighted by
+exp(score / 0.1)), with hard negatives mined from the training database by
+the encoder itself.
 
 Every database sequence is embedded once into an HBM-resident table of
 256 FP8 (e4m3) values per sequence (51.2 GB for 200 M sequences).  A batch
@@ -28,6 +31,7 @@ prefilter's stage-1 selection.
 """
 from __future__ import annotations
 
+import json
 import math
 import time
 from dataclasses import dataclass
@@ -279,6 +283,7 @@ class TrainingSet:
     t_ids: np.ndarray     # int64 [npair] database id of the target
     y: np.ndarray         # float32 [npair] SW(q, t) / SW(q, q)
     homolog: np.ndarray   # bool [npair] the target is a planted homolog of the query
+    selfs: np.ndarray | None = None  # float32 [nquery] SW(q, q)
 
 
 def make_training_set(n_db: int, n_queries: int, randoms_per_query: int, seed: int = 11,
@@ -313,10 +318,23 @@ def make_training_set(n_db: int, n_queries: int, randoms_per_query: int, seed: i
         cand.append(c[srt])
         ishom.append((np.arange(c.size) < hom.size)[srt])
         qidx.append(np.full(c.size, i, np.int32))
-    # exact SW of every pair with the batched SW queue
+    selfs = np.array([float(sw(q, [q])[0]) for q in queries], dtype=np.float32)
+    scores = sw_pairs(db, queries, cand)
+    q_all = np.concatenate(qidx)
+    y = np.clip(scores / np.maximum(selfs[q_all], 1.0), 0.0, 1.0).astype(np.float32)
+    ts = TrainingSet(db, queries, q_all, np.concatenate(cand), y, np.concatenate(ishom))
+    ts.selfs = selfs
+    return ts
+
+
+def sw_pairs(db, queries: list[str], cand: list[np.ndarray]) -> np.ndarray:
+    """Affine SW (11/1, reference stripe rule) of query i against database
+    sequences cand[i] (ascending ids), for every i, with the batched SW
+    queue; the scores of all pairs concatenated (float32)."""
+    from .multi import MultiSearch
     ms = MultiSearch(db, queries, top_prefilter=1, top_hits=1, backtrack=False, tuning={})
     counts = np.array([c.size for c in cand], dtype=np.int64)
-    c_all = np.concatenate(cand)
+    c_all = np.concatenate(cand).astype(np.int64)
     ms.kc = int(counts.max())
     ms.cand = torch.from_numpy(c_all).to(db.device)
     ms.cs = torch.empty(c_all.size, dtype=torch.int32, device=db.device)
@@ -324,24 +342,66 @@ def make_training_set(n_db: int, n_queries: int, randoms_per_query: int, seed: i
     ms.set_sw_tuning({})
     # per-query candidate counts differ: items past a query's list are empty
     ms._sw()
-    scores = ms.cs.cpu().numpy().astype(np.float32)
-    selfs = np.array([float(sw(q, [q])[0]) for q in queries], dtype=np.float32)
-    q_all = np.concatenate(qidx)
-    y = np.clip(scores / np.maximum(selfs[q_all], 1.0), 0.0, 1.0).astype(np.float32)
-    return TrainingSet(db, queries, q_all, c_all, y, np.concatenate(ishom))
+    return ms.cs.cpu().numpy().astype(np.float32)
+
+
+def mine_hard(enc: Encoder, ts: TrainingSet, per_query: int, chunk: int = 128,
+              log=None) -> tuple[list[np.ndarray], list[np.ndarray]]:
+    """Hard negatives: for every training query, the ``per_query`` database
+    sequences that are not its planted homologs and have the highest
+    proposal score under the current encoder, with their normalised SW
+    scores.  Returns (ids per query, y per query)."""
+    from .ops import topk_segments
+    t0 = time.time()
+    tab = EmbeddingTable(enc, ts.db)
+    prop = Proposer(tab)
+    nq, n = len(ts.queries), ts.db.n
+    lo = np.searchsorted(ts.q_ids, np.arange(nq))
+    hi = np.searchsorted(ts.q_ids, np.arange(nq), side="right")
+    ids = []
+    for c0 in range(0, nq, chunk):
+        c1 = min(nq, c0 + chunk)
+        sc = prop.scores_for(ts.queries[c0:c1])
+        for j, i in enumerate(range(c0, c1)):
+            own = ts.t_ids[lo[i]:hi[i]][ts.homolog[lo[i]:hi[i]]]
+            if own.size:
+                sc[j, torch.from_numpy(own).to(sc.device)] = float("-inf")
+        sel, base = topk_segments(sc, np.arange(c1 - c0) * n, np.full(c1 - c0, n),
+                                  np.full(c1 - c0, per_query))
+        sel = sel.cpu().numpy()
+        ids += [sel[base[j]: base[j] + per_query] for j in range(c1 - c0)]
+        del sc
+    del prop, tab
+    torch.cuda.empty_cache()
+    scores = sw_pairs(ts.db, ts.queries, ids)
+    ys = np.clip(scores.reshape(nq, per_query) / np.maximum(ts.selfs[:, None], 1.0), 0.0, 1.0)
+    if log:
+        log(f"[mine] {per_query} hard negatives per query in {time.time() - t0:.1f} s; "
+            f"mean y {float(ys.mean()):.3f}, max y {float(ys.max()):.3f}")
+    return ids, [ys[i].astype(np.float32) for i in range(nq)]
 
 
 def train(enc: Encoder, ts: TrainingSet, steps: int, queries_per_step: int = 32,
           pairs_per_query: int = 32, lr: float = 1e-3, seed: int = 0, log=None,
           time_budget_s: float | None = None, homolog_weight: float = 4.0,
-          lr_floor: float = 0.05) -> list[float]:
-    """AdamW on the squared error between cosine and normalised SW score,
-    homolog pairs weighted ``homolog_weight`` (the proposal must rank the
-    homologs of a query among themselves, background pairs only need to
-    stay low).  Each step takes queries_per_step queries and pairs_per_query
+          lr_floor: float = 0.05, mine_every: int = 0, hard_per_query: int = 64,
+          hard_frac: float = 0.5, mse_weight: float = 1.0, nce_weight: float = 0.0,
+          tau: float = 0.05, tau_y: float = 0.1) -> list[float]:
+    """AdamW on ``mse_weight`` times the squared error between cosine and
+    normalised SW score, homolog pairs weighted ``homolog_weight`` (the
+    proposal must rank the homologs of a query among themselves, background
+    pairs only need to stay low), plus ``nce_weight`` times a soft-target
+    InfoNCE loss: each query's logits are its cosines / ``tau`` with every
+    target of the step (the other queries' targets are its negatives), and
+    its target distribution weights its own pairs by exp(y / ``tau_y``), so
+    that the strongest homologs are pushed above everything else in the
+    step.  Each step takes queries_per_step queries and pairs_per_query
     of each query's pairs, half homologs and half background where
-    available.  The learning rate warms up over 200 steps, then follows a
-    cosine to lr_floor * lr at the last step."""
+    available.  With ``mine_every`` > 0, every mine_every steps the
+    ``hard_per_query`` highest-scoring non-homologs of each query under the
+    current encoder are mined (``mine_hard``) and then fill ``hard_frac`` of
+    the background slots.  The learning rate warms up over 200 steps, then
+    follows a cosine to lr_floor * lr at the last step."""
     rng = np.random.default_rng(seed)
     nq = len(ts.queries)
     # pairs are stored query by query: each query's range, split by kind
@@ -349,21 +409,34 @@ def train(enc: Encoder, ts: TrainingSet, steps: int, queries_per_step: int = 32,
     hi = np.searchsorted(ts.q_ids, np.arange(nq), side="right")
     by_q_h = [lo[i] + np.nonzero(ts.homolog[lo[i]:hi[i]])[0] for i in range(nq)]
     by_q_b = [lo[i] + np.nonzero(~ts.homolog[lo[i]:hi[i]])[0] for i in range(nq)]
-    w_all = np.where(ts.homolog, homolog_weight, 1.0).astype(np.float32)
+    hard_ids = hard_y = None
     losses = []
     t0 = time.time()
     half = pairs_per_query // 2
 
     def pick(i):
+        """(target ids, y, weights) of query i's pairs this step."""
         h = rng.choice(by_q_h[i], size=min(half, by_q_h[i].size), replace=False)
         nb = pairs_per_query - h.size
-        b = rng.choice(by_q_b[i], size=min(nb, by_q_b[i].size), replace=False)
-        return np.concatenate([h, b])
+        nh = int(round(nb * hard_frac)) if hard_ids is not None else 0
+        b = rng.choice(by_q_b[i], size=min(nb - nh, by_q_b[i].size), replace=False)
+        t = [ts.t_ids[h], ts.t_ids[b]]
+        y = [ts.y[h], ts.y[b]]
+        w = [np.full(h.size, homolog_weight, np.float32), np.ones(b.size, np.float32)]
+        if nh:
+            k = rng.choice(hard_ids[i].size, size=min(nh, hard_ids[i].size), replace=False)
+            t.append(hard_ids[i][k])
+            y.append(hard_y[i][k])
+            w.append(np.ones(k.size, np.float32))
+        return np.concatenate(t), np.concatenate(y), np.concatenate(w)
 
     for step in range(steps):
+        if mine_every > 0 and step > 0 and step % mine_every == 0:
+            hard_ids, hard_y = mine_hard(enc, ts, hard_per_query, log=log)
         qs = rng.choice(nq, size=min(queries_per_step, nq), replace=False)
-        pair_idx = np.concatenate([pick(i) for i in qs])
-        tids = ts.t_ids[pair_idx]
+        picks = [pick(i) for i in qs]
+        tids = np.concatenate([p[0] for p in picks])
+        qpos = np.concatenate([np.full(p[0].size, k, np.int32) for k, p in enumerate(picks)])
         # sequences of the step: the queries, then the targets
         qcode, qmask, qrows = enc.rows_of_strings([ts.queries[i] for i in qs])
         tcode, tmask, trows = enc.rows_of_db(ts.db, tids)
@@ -374,15 +447,28 @@ def train(enc: Encoder, ts: TrainingSet, steps: int, queries_per_step: int = 32,
                     np.concatenate([qrows.row0_h, trows.row0_h + qrows.M]),
                     np.concatenate([qrows.len_h, trows.len_h]))
         e, znorm, buf = enc.forward(code, mask, rows, train=True)
-        pos = {q: k for k, q in enumerate(qs)}
-        qi = torch.from_numpy(np.array([pos[q] for q in ts.q_ids[pair_idx]], np.int32)).to(enc.dev)
+        qi = torch.from_numpy(qpos).to(enc.dev)
         ti = torch.arange(len(qs), len(qs) + tids.size, dtype=torch.int32, device=enc.dev)
-        y = torch.from_numpy(ts.y[pair_idx]).to(enc.dev)
-        wt = torch.from_numpy(w_all[pair_idx]).to(enc.dev)
+        y = torch.from_numpy(np.concatenate([p[1] for p in picks])).to(enc.dev)
+        wt = torch.from_numpy(np.concatenate([p[2] for p in picks])).to(enc.dev)
         de = torch.zeros_like(e)
         loss = torch.zeros(1, dtype=torch.float32, device=enc.dev)
-        call("sseqs_enc_pair_loss", ptr(e), int(e.shape[0]), ptr(qi), ptr(ti), ptr(y), ptr(wt),
-             tids.size, ptr(de), None, ptr(loss), stream_of(enc.dev))
+        st = stream_of(enc.dev)
+        if mse_weight > 0:
+            wt *= mse_weight
+            call("sseqs_enc_pair_loss", ptr(e), int(e.shape[0]), ptr(qi), ptr(ti), ptr(y),
+                 ptr(wt), tids.size, ptr(de), None, ptr(loss), st)
+        if nce_weight > 0:
+            P = np.zeros((len(qs), tids.size), np.float32)
+            c0 = 0
+            for k, p in enumerate(picks):
+                w = np.exp((p[1] - p[1].max()) / tau_y)
+                P[k, c0: c0 + w.size] = w / w.sum()
+                c0 += w.size
+            Pd = torch.from_numpy(P).to(enc.dev)
+            G = torch.empty_like(Pd)
+            call("sseqs_enc_infonce", ptr(e), len(qs), tids.size, ptr(Pd), float(tau),
+                 nce_weight / len(qs), ptr(G), ptr(de), ptr(loss), st)
         enc.backward(de, e, znorm, code, mask, rows, buf)
         warm = min(1.0, (step + 1) / 200)
         cos = lr_floor + (1 - lr_floor) * 0.5 * (1 + math.cos(math.pi * step / max(1, steps - 1)))
@@ -394,6 +480,54 @@ def train(enc: Encoder, ts: TrainingSet, steps: int, queries_per_step: int = 32,
         if time_budget_s is not None and time.time() - t0 > time_budget_s:
             break
     return losses
+
+
+def recall_eval(enc: Encoder, n_db: int = 40_000_000, n_eval: int = 32, seed: int = 5,
+                query_seed: int = 7, log=None) -> dict:
+    """Recall of the learned proposal on a scaled copy of the batched
+    workload: the benchmark database construction (synth.bench_db) with
+    n_db sequences and its own seeds, so that each batched query owns the
+    same fraction of homologs, and top-k and the candidate count scaled by
+    s = n_db / 200 M (k = 4096 s, K = 100 000 s).  For n_eval batched queries
+    (lengths uniform in [50, 1000]): recall of the exhaustive SW top-k among
+    the top K proposals, the candidate count that reaches 0.99 recall, and
+    what the top K hold (own homologs, homologs of the two large families of
+    the database, other homologs, background)."""
+    from . import synth
+    s = n_db / 200e6
+    k, K = max(64, int(round(4096 * s))), int(round(100_000 * s))
+    db, _, _, batch = synth.bench_db(n_db, device=enc.dev, seed=seed, query_seed=query_seed)
+    tab = EmbeddingTable(enc, db)
+    prop = Proposer(tab)
+    owner = db.extra["hom_owner"]
+    need = -(-99 * k // 100)
+    per = []
+    for i in np.unique(np.linspace(0, len(batch) - 1, n_eval).round().astype(int)):
+        q = batch[i]
+        kth, allsc = exhaustive_top(db, q, k)
+        good = allsc >= kth
+        sc = prop.scores_for([q])[0]
+        order = torch.sort(sc.float(), descending=True, stable=True).indices
+        cum = torch.cumsum(good[order].to(torch.int32), 0)
+        rec = min(k, int(cum[K - 1].item())) / k
+        k99 = int(torch.searchsorted(cum, torch.tensor([need], dtype=cum.dtype,
+                                                       device=cum.device)).item()) + 1
+        o = owner[order[:K]]
+        per.append({"q_len": len(q), "recall": round(rec, 4), "k99_frac": k99 / db.n,
+                    "own": int((o == 2 + i).sum()), "family": int(((o == 0) | (o == 1)).sum()),
+                    "other_hom": int(((o >= 2) & (o != 2 + i)).sum()), "background": int((o < 0).sum())})
+        del order, cum, allsc, good, sc
+        if log:
+            log(f"[eval] {json.dumps(per[-1])}")
+    del prop, tab, db
+    torch.cuda.empty_cache()
+    rec = np.array([p["recall"] for p in per])
+    kf = np.array([p["k99_frac"] for p in per])
+    return {"n_db": n_db, "k": k, "K": K, "queries": len(per),
+            "recall_mean": round(float(rec.mean()), 4), "recall_min": round(float(rec.min()), 4),
+            "recall_p10": round(float(np.quantile(rec, 0.1)), 4),
+            "k99_frac_median": float(np.median(kf)), "k99_frac_mean": float(kf.mean()),
+            "k99_frac_max": float(kf.max()), "per_query": per}
 
 
 # --------------------------------------------------------------- table --

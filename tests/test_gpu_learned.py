@@ -9,7 +9,9 @@ definition.
 * backward: the gradients of a pair loss with respect to every parameter
   block equal autograd's of the fp32 model within the error that bf16
   operands bring (relative L2 error below 5e-2, cosine above 0.998);
-* the training step moves the loss down;
+* the weighted pair loss and the soft-target InfoNCE loss (value and
+  gradient) equal fp64 autograd of their formulas;
+* the training step moves the loss down, with either loss;
 * the FP8 table holds e4m3 of 16 * e, and rows built in chunks equal the
   same sequences embedded in one batch, bit for bit;
 * proposal scores equal the fp32 dot products of the fp8 operands within
@@ -122,6 +124,49 @@ def test_encoder_backward_matches_autograd(enc, dev):
         assert rel < 5e-2 and cosg > 0.998, (name, rel, cosg)
 
 
+def test_pair_and_infonce_losses_match_reference(dev):
+    """Weighted pair loss and soft-target InfoNCE on random unit rows against
+    autograd of the same formulas in fp64 on the CPU."""
+    from sseqs_amd._lib import call, ptr, stream_of
+    announce("weighted pair loss and InfoNCE vs fp64 autograd")
+    rng = np.random.default_rng(4)
+    nq, nt, tau = 7, 300, 0.07
+    e64 = torch.from_numpy(rng.standard_normal((nq + nt, 256)))
+    e64 = e64 / e64.norm(dim=1, keepdim=True)
+    P = rng.uniform(0, 1, (nq, nt)) * (rng.uniform(0, 1, (nq, nt)) < 0.1)
+    P[2] = 0.0  # a query without a distribution takes no part
+    P = P / np.maximum(P.sum(1, keepdims=True), 1e-30)
+    qi = rng.integers(0, nq, 50).astype(np.int32)
+    ti = (nq + rng.integers(0, nt, 50)).astype(np.int32)
+    y = rng.uniform(0, 1, 50).astype(np.float32)
+    w = rng.uniform(0.5, 4, 50).astype(np.float32)
+    active = int((P.sum(1) > 0).sum())
+    scale = 0.6 / active
+    # held in names until the launches have been enqueued
+    e = e64.float().to(dev)
+    Pd = torch.from_numpy(P.astype(np.float32)).to(dev)
+    qd, td, yd, wd = (torch.from_numpy(x).to(dev) for x in (qi, ti, y, w))
+    G = torch.empty((nq, nt), dtype=torch.float32, device=dev)
+    de = torch.zeros_like(e)
+    loss = torch.zeros(1, dtype=torch.float32, device=dev)
+    call("sseqs_enc_pair_loss", ptr(e), nq + nt, ptr(qd), ptr(td), ptr(yd), ptr(wd), 50, ptr(de),
+         None, ptr(loss), stream_of(dev))
+    call("sseqs_enc_infonce", ptr(e), nq, nt, ptr(Pd), tau, scale, ptr(G), ptr(de), ptr(loss),
+         stream_of(dev))
+    torch.cuda.synchronize()
+    x = e.double().cpu().requires_grad_(True)
+    cos = (x[torch.from_numpy(qi).long()] * x[torch.from_numpy(ti).long()]).sum(1)
+    mse = (torch.from_numpy(w).double() * (cos - torch.from_numpy(y).double()) ** 2).mean()
+    logp = torch.log_softmax(x[:nq] @ x[nq:].T / tau, dim=1)
+    nce = -scale * (torch.from_numpy(P) * logp).sum()
+    (mse + nce).backward()
+    want = (mse + nce).item()
+    assert abs(loss.item() - want) < 1e-4 * max(1.0, abs(want)), (loss.item(), want)
+    err = (de.cpu().double() - x.grad).abs().max().item()
+    assert err < 1e-4 * x.grad.abs().max().item() + 1e-6, err
+    assert G[2].abs().max().item() == 0.0
+
+
 def test_training_lowers_the_loss(dev):
     from sseqs_amd.learned import Encoder, make_training_set, train
     announce("encoder training on synthetic pairs")
@@ -132,6 +177,10 @@ def test_training_lowers_the_loss(dev):
     enc = Encoder(dev, seed=4)
     losses = train(enc, ts, steps=120, queries_per_step=16, pairs_per_query=16, lr=2e-3)
     assert np.mean(losses[-2:]) < 0.7 * losses[0], losses
+    enc = Encoder(dev, seed=4)
+    losses = train(enc, ts, steps=120, queries_per_step=16, pairs_per_query=16, lr=2e-3,
+                   mse_weight=0.0, nce_weight=1.0)
+    assert np.mean(losses[-2:]) < 0.8 * losses[0], losses
 
 
 def test_table_and_proposal(orc, enc, dev):
