@@ -214,7 +214,7 @@ def main() -> int:
                     help="of those, how many get the plain cascade timed at its 0.99 point")
     ap.add_argument("--serve-requests", type=int, default=400,
                     help="requests of the serving measurement at the high rate (0 skips it)")
-    ap.add_argument("--serve-plain-requests", type=int, default=16)
+    ap.add_argument("--serve-plain-requests", type=int, default=8)
     ap.add_argument("--serve-rate-high", type=float, default=200.0)
     ap.add_argument("--serve-rate-low", type=float, default=20.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -585,7 +585,8 @@ def main() -> int:
         the whole database for a sample of the queries, for both
         prefilters, with each one's candidate count for 0.99 recall and the
         time of the cascade run there."""
-        from sseqs_amd.learned import WEIGHTS_PATH, EmbeddingTable, Encoder, Proposer, exhaustive_top
+        from sseqs_amd.learned import (WEIGHTS_PATH, EmbeddingTable, Encoder, Proposer,
+                                       exhaustive_top, recall_curve)
         from sseqs_amd.search import global_search
         if not WEIGHTS_PATH.exists():
             log("no trained encoder weights; learned prefilter skipped")
@@ -621,16 +622,9 @@ def main() -> int:
             gh = global_search(db, q, args.top_prefilter, k, scores_buf=scores_buf)
             r["plain_recall_at_top_prefilter"] = round(
                 min(k, int((gh.scores >= kth).sum().item())) / k, 4)
-            need = -(-99 * k // 100)
             for name, sc in (("plain", gh.prefilter), ("learned", prop.scores_for([qtext])[0])):
-                order = torch.sort(sc.float(), descending=True, stable=True).indices
-                cum = torch.cumsum(good[order].to(torch.int32), 0)
-                del order
-                r[f"{name}_recall_curve"] = {str(K): round(min(k, int(cum[K - 1].item())) / k, 4)
-                                             for K in grid if K <= db.n}
-                K99 = int(torch.searchsorted(cum, torch.tensor([need], dtype=cum.dtype,
-                                                               device=dev)).item()) + 1
-                del cum
+                curve, K99 = recall_curve(sc, good, k, grid)
+                r[f"{name}_recall_curve"] = {str(K): v for K, v in curve.items()}
                 r[f"{name}_top_prefilter_for_0.99"] = K99
             # each cascade at its own 0.99 operating point, timed (for the
             # first --recall-timed queries of the sample)

@@ -326,6 +326,25 @@ int sseqs_enc_to_fp8(const float* e, int64_t n, float scale, uint8_t* out, hipSt
 int sseqs_propose(const uint8_t* table, int64_t n, const uint8_t* q8, int G, void* scores,
                   int64_t ld, hipStream_t stream);
 
+/* ------------------------------------------------------- a3m (host) -- */
+
+/* Bytes of output that sseqs_a3m_text needs at most for nrow rows whose
+ * aligned lengths sum to aligned_total (-1 for a negative argument). */
+int64_t sseqs_a3m_bound(int64_t Lq, int64_t nrow, int64_t aligned_total);
+/* A3M text of one query's backtracked hits, host memory only
+ * (sseqs/to_a3m.py:16-72, rows as sseqs/to_a3m.py:23-49).  Row b's aligned
+ * query and target bytes are h_q / h_t + h_row_off[b], h_row_len[b] long,
+ * its score h_score[b]; rows are taken in the order h_order (a permutation
+ * of 0..nrow-1, the reference's np.argsort(scores)[::-1]).  Writes the text
+ * (not terminated) to h_out and returns its length; -1 when an aligned
+ * query fragment is not in the query (the reference raises), -2 when the
+ * query or a row holds a byte >= 128 (the caller formats those), -3 when
+ * cap < sseqs_a3m_bound(...), -4 for a bad argument. */
+int64_t sseqs_a3m_text(const char* h_query, int64_t Lq, const uint8_t* h_q, const uint8_t* h_t,
+                       const int64_t* h_row_off, const int32_t* h_row_len,
+                       const int32_t* h_score, const int64_t* h_order, int64_t nrow,
+                       char* h_out, int64_t cap);
+
 /* --------------------------------------------------- synthetic database -- */
 
 /* Parameters of the synthetic generator; layout shared with

@@ -38,6 +38,8 @@ class GenParams(C.Structure):
 SIGNATURES = {
     "sseqs_last_error": (C.c_char_p, []),
     "sseqs_abi_version": (_i32, []),
+    "sseqs_a3m_bound": (_i64, [_i64, _i64, _i64]),
+    "sseqs_a3m_text": (_i64, [C.c_char_p, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _i64, _vp, _i64]),
     "decompress_launcher": (_i32, [_vp, _vp, _i64, _i64, _vp]),
     "compress_launcher": (_i32, [_vp, _vp, _i64, _vp]),
     "launch_warp_corr": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _i32, _i32,

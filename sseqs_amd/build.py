@@ -1,4 +1,6 @@
-"""Builds sseqs_amd/libsseqs_amd.so from csrc/*.hip with hipcc for gfx950.
+"""Builds sseqs_amd/libsseqs_amd.so from csrc/*.hip (hipcc, gfx950) and the
+host-only csrc/*.cpp (the host compiler; torch_ext.cpp is the separate
+PyTorch extension).
 
 Run as ``python -m sseqs_amd.build`` (or through ``__graft_entry__.build()``).
 Objects are compiled in parallel into ``sseqs_amd/_build/`` and linked into
@@ -35,9 +37,17 @@ FLAGS = [
 ]
 
 
+# host-only sources of the library (everything but the PyTorch extension)
+HOST_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-D__HIP_PLATFORM_AMD__=1",
+              "-I/opt/rocm/include", f"-I{ROOT / 'include'}"]
+
+
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.stem + ".o")
-    cmd = [HIPCC, *FLAGS, "-c", str(src), "-o", str(obj)]
+    if src.suffix == ".cpp":
+        cmd = [os.environ.get("CXX", "g++"), *HOST_FLAGS, "-c", str(src), "-o", str(obj)]
+    else:
+        cmd = [HIPCC, *FLAGS, "-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
     if r.returncode != 0:
@@ -84,6 +94,7 @@ def build(verbose: bool = True) -> Path:
     srcs = sorted(CSRC.glob("*.hip"))
     if not srcs:
         raise RuntimeError("no HIP sources found")
+    srcs += sorted(p for p in CSRC.glob("*.cpp") if p != EXT_SRC)
     jobs = min(len(srcs), int(os.environ.get("MAX_JOBS", "8")))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(_compile, srcs))

@@ -42,6 +42,7 @@ import torch
 
 from . import _lib
 from ._lib import call, ptr, stream_of
+from .ops import topk_indices
 from .query import AA_MAP
 
 WEIGHTS_PATH = Path(__file__).resolve().parent / "encoder_weights.npz"
@@ -500,23 +501,17 @@ def recall_eval(enc: Encoder, n_db: int = 40_000_000, n_eval: int = 32, seed: in
     tab = EmbeddingTable(enc, db)
     prop = Proposer(tab)
     owner = db.extra["hom_owner"]
-    need = -(-99 * k // 100)
     per = []
     for i in np.unique(np.linspace(0, len(batch) - 1, n_eval).round().astype(int)):
         q = batch[i]
         kth, allsc = exhaustive_top(db, q, k)
-        good = allsc >= kth
         sc = prop.scores_for([q])[0]
-        order = torch.sort(sc.float(), descending=True, stable=True).indices
-        cum = torch.cumsum(good[order].to(torch.int32), 0)
-        rec = min(k, int(cum[K - 1].item())) / k
-        k99 = int(torch.searchsorted(cum, torch.tensor([need], dtype=cum.dtype,
-                                                       device=cum.device)).item()) + 1
-        o = owner[order[:K]]
-        per.append({"q_len": len(q), "recall": round(rec, 4), "k99_frac": k99 / db.n,
+        curve, k99 = recall_curve(sc, allsc >= kth, k, (K,))
+        o = owner[topk_indices(sc, K, sorted=False)]
+        per.append({"q_len": len(q), "recall": curve[K], "k99_frac": k99 / db.n,
                     "own": int((o == 2 + i).sum()), "family": int(((o == 0) | (o == 1)).sum()),
                     "other_hom": int(((o >= 2) & (o != 2 + i)).sum()), "background": int((o < 0).sum())})
-        del order, cum, allsc, good, sc
+        del allsc, sc, o
         if log:
             log(f"[eval] {json.dumps(per[-1])}")
     del prop, tab, db
@@ -618,6 +613,32 @@ class Proposer:
         call("sseqs_propose", ptr(tab.table), tab.n, ptr(q8), Gp, ptr(out), tab.n, st)
         return out[: len(queries)]
 
+    def calibrate(self, queries: list[str], top_prefilter: int, k: int = 4096, log=None) -> dict:
+        """Recall@k of the proposal against exhaustive SW of the whole
+        database for sample queries: per query, the recall among the top
+        ``top_prefilter`` proposals and the candidate count that reaches the
+        gate.  ``use_learned`` becomes whether every sample query reaches
+        the gate at ``top_prefilter``; otherwise the plain prefilter
+        selects."""
+        db = self.table.db
+        per = []
+        for q in queries:
+            kth, allsc = exhaustive_top(db, q, k)
+            curve, kg = recall_curve(self.scores_for([q])[0], allsc >= kth, k,
+                                     (top_prefilter,), self.gate)
+            per.append({"q_len": len(q), "kth_exhaustive_score": kth,
+                        "recall": curve.get(top_prefilter, 1.0), "top_prefilter_for_gate": kg})
+            del allsc
+            if log:
+                log(f"[calibrate] {json.dumps(per[-1])}")
+        rec = [p["recall"] for p in per]
+        self.use_learned = bool(per) and min(rec) >= self.gate
+        self.calibration = {"queries": len(per), "top_prefilter": top_prefilter, "k": k,
+                            "gate": self.gate, "recall_min": min(rec, default=None),
+                            "recall_mean": float(np.mean(rec)) if rec else None,
+                            "use_learned": self.use_learned, "per_query": per}
+        return self.calibration
+
     def prepare(self, ms) -> None:
         """Set-up of a sweep (before any capture): the queries' fp8
         embedding rows, in groups of at most ``group`` (each padded to a
@@ -664,6 +685,24 @@ class Proposer:
                      ms.sel_ws.numel(), st)
 
 
+def recall_curve(scores: torch.Tensor, good: torch.Tensor, k: int, grid=(),
+                 gate: float = 0.99) -> tuple[dict, int]:
+    """Recall@k of a candidate ranking: ``good`` marks the sequences whose
+    exhaustive SW score reaches the k-th best (ties count as found), the
+    candidates are the top K by ``scores`` (stable, index order among equal
+    scores).  Returns ({K: recall} for K in grid, the smallest K whose
+    recall reaches ``gate``)."""
+    order = torch.sort(scores.float(), descending=True, stable=True).indices
+    cum = torch.cumsum(good[order].to(torch.int32), 0)
+    del order
+    curve = {int(K): round(min(k, int(cum[K - 1].item())) / k, 4) for K in grid
+             if 0 < K <= cum.numel()}
+    need = -(-int(round(gate * 10000)) * k // 10000)
+    kg = int(torch.searchsorted(cum, torch.tensor([need], dtype=cum.dtype,
+                                                  device=cum.device)).item()) + 1
+    return curve, min(kg, cum.numel())
+
+
 def exhaustive_top(db, query: str, k: int, chunk: int = 25_000_000) -> tuple[int, torch.Tensor]:
     """k-th best affine SW score of the query over the whole database, and
     every score (int32, index order)."""
@@ -674,5 +713,5 @@ def exhaustive_top(db, query: str, k: int, chunk: int = 25_000_000) -> tuple[int
     for b in range(0, db.n, chunk):
         ids = torch.arange(b, min(db.n, b + chunk), device=db.device)
         allsc[b: b + ids.numel()] = sw_affine_ids(db, q, ids)
-    kth = int(torch.topk(allsc, k).values[-1].item())
+    kth = int(allsc[topk_indices(allsc, k, sorted=False)].min().item())
     return kth, allsc

@@ -38,6 +38,7 @@ from ._lib import call, ptr, stream_of
 from .db import SeqDB
 from .query import PreparedQuery, prepare
 from .search import BacktrackWorkspace, DeviceAlignments, alignments_to_dicts
+from .to_a3m import a3m_text, a3m_text_rows
 
 TUNING_PATH = Path(__file__).resolve().parent / "tuning.json"
 
@@ -94,6 +95,14 @@ class QueryHits:
 
     def alignment_dicts(self) -> list[dict]:
         return alignments_to_dicts(self.alignments) if self.alignments is not None else []
+
+    def a3m_text(self) -> str:
+        """The a3m text of the hits (``to_a3m.a3m_text_rows`` of the rows)."""
+        al = self.alignments
+        if al is None:
+            return a3m_text(self.query, [])
+        return a3m_text_rows(self.query, [(al.out_q.cpu().numpy(), al.out_t.cpu().numpy(),
+                                           al.info.cpu().numpy())])
 
 
 class MultiSearch:

@@ -12,8 +12,10 @@ proposal, SW, hits and backtrack of every query of the sweep):
 * overlap: a sweep runs in two stages on two streams (of the same, default
   priority) -- candidate selection on the first, SW + hits + backtrack on
   the second -- so the next sweep's selection starts while the previous
-  sweep's SW and backtrack still run, and the host formats a finished
-  sweep's a3m text on a worker thread while the device works on the next;
+  sweep's SW and backtrack still run, and the host formats finished
+  sweeps' a3m text (the library's host formatter, which releases the
+  interpreter lock) on two worker threads while the device works on the
+  next;
   at most ``in_flight`` sweeps hold buffers at once, so a sweep is admitted
   when its predecessor frees them;
 * a query's hits, alignments and a3m text do not depend on which other
@@ -38,7 +40,6 @@ import torch
 from . import _lib
 from .db import SeqDB
 from .multi import MultiSearch
-from .to_a3m import a3m_text
 
 
 @dataclass
@@ -102,7 +103,7 @@ class SearchEngine:
         self._slots = threading.Semaphore(self.in_flight)
         self._stop = False
         self._streams = [torch.cuda.Stream(db.device), torch.cuda.Stream(db.device)]
-        self._fin = cf.ThreadPoolExecutor(max_workers=1, thread_name_prefix="sseqs-finish")
+        self._fin = cf.ThreadPoolExecutor(max_workers=2, thread_name_prefix="sseqs-finish")
         self._loop = threading.Thread(target=self._run, name="sseqs-device-loop", daemon=True)
         self._loop.start()
 
@@ -205,7 +206,7 @@ class SearchEngine:
             res = ms.results()
             texts = []
             for r, h in zip(batch, res):
-                texts.append(a3m_text(r.query, h.alignment_dicts()))
+                texts.append(h.a3m_text())
         except Exception as e:  # noqa: BLE001 - delivered to the callers
             for r in batch:
                 r.future.set_exception(e)

@@ -37,12 +37,15 @@ def sw(query: str, targets: list[str], gap_open: int = 11, gap_extend: int = 1) 
 
 
 def sw_affine_backtrack(query: str, targets: list[str], gap_open: int = 11, gap_extend: int = 1,
-                        device: str = "cuda", scores: bool = False):
+                        device: str = "cuda", scores: bool = False, rows: bool = False):
     """Local affine alignment of every target (sseqs/sw.py:27-219).
 
     Returns ``(None, best_scores_int32, aligned)`` -- the reference's first
     element is the full int32 M matrix of the batch, which this kernel never
-    materialises -- or only ``best_scores_int32`` when ``scores=True``.
+    materialises -- or only ``best_scores_int32`` when ``scores=True``, or
+    with ``rows=True`` the host arrays (q rows, t rows, info) that the
+    dicts are made from: target b's alignment is the last info[b, 0] bytes
+    of its rows, info[b] = (length, q_start, q_end, t_start, t_end, score).
     Each ``aligned`` entry is a dict with q_aligned, t_aligned,
     q_start_orig_0based, q_end_orig_0based, t_start_orig_0based,
     t_end_orig_0based and score (float).
@@ -75,6 +78,8 @@ def sw_affine_backtrack(query: str, targets: list[str], gap_open: int = 11, gap_
     best = info[:, 5].clone()
     if scores:
         return best
+    if rows:
+        return out_q.cpu().numpy(), out_t.cpu().numpy(), info.cpu().numpy()
     qh, th_, ih = out_q.cpu().numpy(), out_t.cpu().numpy(), info.cpu().numpy()
     aligned = []
     for b in range(B):

@@ -75,14 +75,73 @@ def a3m_text(query: str, aligned: list[dict]) -> str:
     return "".join(out)
 
 
+def _dicts(blocks) -> list[dict]:
+    """Backtrack dicts (sseqs/sw.py:207-217) of row blocks."""
+    out = []
+    for qh, th_, ih in blocks:
+        A = qh.shape[1]
+        for b in range(ih.shape[0]):
+            n = int(ih[b, 0])
+            out.append({
+                "q_aligned": qh[b, A - n:].tobytes().decode("latin-1") if n else "",
+                "t_aligned": th_[b, A - n:].tobytes().decode("latin-1") if n else "",
+                "q_start_orig_0based": int(ih[b, 1]), "q_end_orig_0based": int(ih[b, 2]),
+                "t_start_orig_0based": int(ih[b, 3]), "t_end_orig_0based": int(ih[b, 4]),
+                "score": float(ih[b, 5])})
+    return out
+
+
+def a3m_text_rows(query: str, blocks) -> str:
+    """``a3m_text`` of backtrack rows without building dicts: ``blocks`` is a
+    list of (q rows uint8 [B, A], t rows uint8 [B, A], info int32 [B, 6]),
+    each alignment right-aligned in its rows (the device backtrack's
+    layout).  The rows are formatted by the library's host routine
+    ``sseqs_a3m_text`` in the order a3m_text uses (numpy's argsort of the
+    float scores, reversed); a query or row that is not ASCII goes through
+    a3m_text."""
+    from . import _lib
+    blocks = [(np.ascontiguousarray(q, np.uint8), np.ascontiguousarray(t, np.uint8),
+               np.ascontiguousarray(i, np.int32)) for q, t, i in blocks]
+    qb = query.encode("latin-1", errors="replace")
+    if not blocks or not query.isascii():
+        return a3m_text(query, _dicts(blocks))
+    if len(blocks) == 1:
+        Q, T, info = blocks[0][0].reshape(-1), blocks[0][1].reshape(-1), blocks[0][2]
+    else:
+        Q = np.concatenate([q.reshape(-1) for q, _, _ in blocks])
+        T = np.concatenate([t.reshape(-1) for _, t, _ in blocks])
+        info = np.concatenate([i for _, _, i in blocks])
+    n = np.ascontiguousarray(info[:, 0])
+    base, offs = 0, []
+    for q, _, i in blocks:
+        A = q.shape[1]
+        offs.append(base + np.arange(i.shape[0], dtype=np.int64) * A + (A - i[:, 0]))
+        base += q.size
+    off = np.ascontiguousarray(np.concatenate(offs).astype(np.int64))
+    score = np.ascontiguousarray(info[:, 5])
+    order = np.ascontiguousarray(np.argsort(score.astype(np.float64))[::-1].astype(np.int64))
+    lib = _lib.load()
+    cap = int(lib.sseqs_a3m_bound(len(qb), n.size, int(np.maximum(n, 0).sum())))
+    out = np.empty(max(cap, 1), dtype=np.uint8)
+    p = lambda a: a.ctypes.data
+    r = int(lib.sseqs_a3m_text(qb, len(qb), p(Q), p(T), p(off), p(n), p(score), p(order), n.size,
+                               p(out), cap))
+    if r == -2:
+        return a3m_text(query, _dicts(blocks))
+    if r == -1:
+        raise ValueError("Cannot locate aligned fragment in query.")
+    if r < 0:
+        raise _lib.SseqsError(f"sseqs_a3m_text failed ({r})")
+    return out[:r].tobytes().decode("ascii")
+
+
 def to_a3m(query: str, seqs: list[str], filename: str, gap_open: int = 11,
            gap_extend: int = 1) -> str:
     """Write the a3m file for ``seqs`` aligned to ``query``; returns the text."""
-    aligned: list[dict] = []
-    for start in range(0, len(seqs), 1024):
-        aligned += sw_affine_backtrack(query, list(seqs[start:start + 1024]),
-                                       gap_open=gap_open, gap_extend=gap_extend)[-1]
-    text = a3m_text(query, aligned)
+    blocks = [sw_affine_backtrack(query, list(seqs[start:start + 1024]), gap_open=gap_open,
+                                  gap_extend=gap_extend, rows=True)
+              for start in range(0, len(seqs), 1024)]
+    text = a3m_text_rows(query, blocks)
     if filename is not None:
         with open(filename, "w") as fh:
             fh.write(text)

@@ -31,7 +31,7 @@ def test_library_exports_every_symbol():
     assert not missing, missing
     # the Python binding table covers the same set
     assert sorted(_lib.SIGNATURES) == header_functions()
-    assert lib.sseqs_abi_version() == 1
+    assert lib.sseqs_abi_version() == 2
 
 
 def test_argument_errors_need_no_gpu():

@@ -16,7 +16,10 @@ definition.
   same sequences embedded in one batch, bit for bit;
 * proposal scores equal the fp32 dot products of the fp8 operands within
   fp16 rounding, and the proposed candidates are the exact top-k of those
-  scores; the learned cascade's SW scores are the oracle's.
+  scores; the learned cascade's SW scores are the oracle's;
+* calibration recall equals the share of the oracle's exhaustive top k
+  among the top K proposals, and a failed gate hands selection back to the
+  plain prefilter.
 """
 import numpy as np
 import pytest
@@ -223,3 +226,25 @@ def test_table_and_proposal(orc, enc, dev):
         cand = ms.cand[i * 600:(i + 1) * 600].cpu().numpy()
         sw = orc.sw_affine_segments(orc.encode_query_sw(q), ascii, off[cand], ln[cand])
         assert np.array_equal(ms.cs[i * 600:(i + 1) * 600].cpu().numpy(), sw)
+    # calibration against exhaustive SW (oracle scores of every sequence):
+    # recall at K is the share of the exhaustive top k among the top K
+    # proposals, and the gate decides between the learned and plain selection
+    prop = ms.proposer
+    k, K = 40, 600
+    cal = prop.calibrate(queries, top_prefilter=K, k=k)
+    for i, q in enumerate(queries):
+        allsc = orc.sw_affine_segments(orc.encode_query_sw(q), ascii, off, ln)
+        kth = np.sort(allsc)[-k]
+        top = orc.topk_desc(sc[i].numpy().astype(np.float64), K)
+        rec = min(k, int((allsc[top] >= kth).sum())) / k
+        assert cal["per_query"][i]["kth_exhaustive_score"] == kth
+        assert abs(cal["per_query"][i]["recall"] - round(rec, 4)) < 1e-9, i
+    assert prop.use_learned == (min(p["recall"] for p in cal["per_query"]) >= 0.99)
+    cal = prop.calibrate(queries, top_prefilter=db.n, k=k)
+    assert prop.use_learned and all(p["recall"] == 1.0 for p in cal["per_query"])
+    prop.calibrate(queries, top_prefilter=1, k=k)
+    assert not prop.use_learned
+    plain = MultiSearch(db, queries, top_prefilter=600, top_hits=50)
+    plain.run()
+    ms.run()  # falls back to the plain prefilter
+    assert torch.equal(ms.cand, plain.cand) and torch.equal(ms.hit_sc, plain.hit_sc)
