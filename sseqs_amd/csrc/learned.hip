@@ -290,27 +290,56 @@ __global__ __launch_bounds__(256) void k_embed_grad(const uint8_t* __restrict__ 
 }
 
 // Sequence s occupies rows [row0[s], row0[s] + len[s]).  z = mean of h3
-// rows, e = z / |z| (|z| = 0 leaves e = 0); one block of 256 threads (one
-// per channel) per sequence.
+// rows, e = z / |z| (|z| = 0 leaves e = 0); one block of 256 threads per
+// sequence.  Thread t reads channels 4 (t % 64) .. +3 of rows p = t / 64
+// (mod 4), 8 bytes per load and 16 rows of the block in flight per step;
+// the four row classes are summed in LDS, in a fixed order.
 __global__ __launch_bounds__(256) void k_pool(const __bf16* __restrict__ h3,
                                               const int64_t* __restrict__ row0,
                                               const int32_t* __restrict__ len, int64_t nseq,
                                               float* __restrict__ e, float* __restrict__ znorm) {
+    __shared__ float part[4][256];
     __shared__ float red[4];
-    const int c = threadIdx.x;
+    const int t = threadIdx.x, c4 = 4 * (t & 63), ph = t >> 6;
     for (int64_t s = blockIdx.x; s < nseq; s += gridDim.x) {
         const int64_t b = row0[s];
         const int L = len[s];
-        float acc = 0.f;
-        for (int p = 0; p < L; ++p) acc += bf2f(h3[(b + p) * 256 + c]);
+        float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+        int p = ph;
+        for (; p + 12 < L; p += 16) {
+            uint2 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                v[u] = *reinterpret_cast<const uint2*>(h3 + (b + p + 4 * u) * 256 + c4);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                a0 += __uint_as_float(v[u].x << 16);
+                a1 += __uint_as_float(v[u].x & 0xffff0000u);
+                a2 += __uint_as_float(v[u].y << 16);
+                a3 += __uint_as_float(v[u].y & 0xffff0000u);
+            }
+        }
+        for (; p < L; p += 4) {
+            const uint2 v = *reinterpret_cast<const uint2*>(h3 + (b + p) * 256 + c4);
+            a0 += __uint_as_float(v.x << 16);
+            a1 += __uint_as_float(v.x & 0xffff0000u);
+            a2 += __uint_as_float(v.y << 16);
+            a3 += __uint_as_float(v.y & 0xffff0000u);
+        }
+        part[ph][c4] = a0;
+        part[ph][c4 + 1] = a1;
+        part[ph][c4 + 2] = a2;
+        part[ph][c4 + 3] = a3;
+        __syncthreads();
+        const float acc = (part[0][t] + part[1][t]) + (part[2][t] + part[3][t]);
         const float z = L > 0 ? acc / (float)L : 0.f;
         float sq = z * z;
         for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
-        if ((c & 63) == 0) red[c >> 6] = sq;
+        if ((t & 63) == 0) red[t >> 6] = sq;
         __syncthreads();
         const float nrm = sqrtf(red[0] + red[1] + red[2] + red[3]);
-        e[s * 256 + c] = nrm > 0.f ? z / nrm : 0.f;
-        if (c == 0 && znorm) znorm[s] = nrm;
+        e[s * 256 + t] = nrm > 0.f ? z / nrm : 0.f;
+        if (t == 0 && znorm) znorm[s] = nrm;
         __syncthreads();
     }
 }
@@ -531,21 +560,31 @@ __global__ void k_to_fp8(const float* __restrict__ e, int64_t n, float scale,
 //
 // S[q][s] = sum_c T[s][c] * Q[q][c] over the fp8 table T (n x 256) and G
 // query rows Q (G x 256, G a multiple of 16, at most 128), written as fp16
-// to S[q * ldS + s].  Block: 128 table rows (4 waves x 32 rows) against
-// all G queries held in LDS; MFMA 16x16x32 fp8: lane (r = lane % 16,
-// g = lane / 16) supplies row r, bytes 8g .. 8g+7 of the 32-byte K slice;
-// result element i of lane (c, g) is row 4g + i, column c.
+// to S[q * ldS + s].  A block takes 128 consecutive table rows per step
+// (4 waves x 32 rows) against all G queries held in LDS.  MFMA 16x16x32
+// fp8: lane (r = lane % 16, g = lane / 16) supplies row r, bytes 8g .. 8g+7
+// of the 32-byte K slice; result element i of lane (c, g) is row 4g + i,
+// column c.  Each group of 16 queries' 16 x 128 scores goes through LDS so
+// that the block writes every query's 256 contiguous bytes with 16-byte
+// stores (the score matrix is a quarter of the traffic).
+constexpr int PROP_ROWS = 128;
+
 __global__ __launch_bounds__(256) void k_propose(const uint8_t* __restrict__ T, int64_t n,
                                                  const uint8_t* __restrict__ Q, int G,
                                                  _Float16* __restrict__ S, int64_t ldS) {
     __shared__ __align__(16) uint8_t sQ[128 * 256];
+    __shared__ __align__(16) _Float16 sS[16 * PROP_ROWS];
     for (int i = threadIdx.x; i < G * 16; i += 256)
         reinterpret_cast<uint4*>(sQ)[i] = reinterpret_cast<const uint4*>(Q)[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r = lane & 15, g = lane >> 4;
-    for (int64_t t0 = ((int64_t)blockIdx.x * 4 + w) * 32; t0 < n;
-         t0 += (int64_t)gridDim.x * 128) {
+    // the cooperative store: thread -> (query oq of the group, rows 8 oc .. 8 oc + 7)
+    const int oq = threadIdx.x >> 4, oc = threadIdx.x & 15;
+    const bool vec = (ldS & 7) == 0 && (reinterpret_cast<uintptr_t>(S) & 15) == 0;
+    for (int64_t b0 = (int64_t)blockIdx.x * PROP_ROWS; b0 < n;
+         b0 += (int64_t)gridDim.x * PROP_ROWS) {
+        const int64_t t0 = b0 + 32 * w;
         // 32 table rows (two MFMA row tiles) of this wave, all 8 K-slices
         long fa[2][8];
 #pragma unroll
@@ -568,13 +607,21 @@ __global__ __launch_bounds__(256) void k_propose(const uint8_t* __restrict__ T, 
                                                                         0, 0);
             }
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-                for (int x = 0; x < 4; ++x) {
-                    const int64_t row = t0 + 16 * i + 4 * g + x;
-                    if (row < n) S[(int64_t)(q0 + r) * ldS + row] = (_Float16)acc[i][x];
-                }
+                for (int x = 0; x < 4; ++x)
+                    sS[r * PROP_ROWS + 32 * w + 16 * i + 4 * g + x] = (_Float16)acc[i][x];
+            __syncthreads();
+            const int64_t row = b0 + 8 * oc;
+            _Float16* dst = S + (int64_t)(q0 + oq) * ldS + row;
+            const _Float16* src = &sS[oq * PROP_ROWS + 8 * oc];
+            if (vec && row + 8 <= n) {
+                *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(src);
+            } else {
+                for (int x = 0; x < 8; ++x)
+                    if (row + x < n) dst[x] = src[x];
             }
+            __syncthreads();
         }
     }
 }

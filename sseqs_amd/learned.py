@@ -582,6 +582,12 @@ class EmbeddingTable:
 
 # ------------------------------------------------------------ proposer --
 
+def _ld(n: int) -> int:
+    """Row stride of a proposal score matrix: n rounded up to 8 scores, so
+    that every row starts 16-byte aligned for the kernel's vector stores."""
+    return (n + 7) // 8 * 8
+
+
 class Proposer:
     """Candidate selection for multi.MultiSearch from the embedding table:
     per query, the top kc sequences by e_q . e_t (FP8 MFMA), in ascending
@@ -609,9 +615,10 @@ class Proposer:
         q[: len(queries)] = e
         q8 = torch.empty((Gp, DIM), dtype=torch.uint8, device=dev)
         call("sseqs_enc_to_fp8", ptr(q), q.numel(), FP8_SCALE, ptr(q8), st)
-        out = torch.empty((Gp, tab.n), dtype=torch.float16, device=dev)
-        call("sseqs_propose", ptr(tab.table), tab.n, ptr(q8), Gp, ptr(out), tab.n, st)
-        return out[: len(queries)]
+        ld = _ld(tab.n)
+        out = torch.empty((Gp, ld), dtype=torch.float16, device=dev)
+        call("sseqs_propose", ptr(tab.table), tab.n, ptr(q8), Gp, ptr(out), ld, st)
+        return out[: len(queries), : tab.n]
 
     def calibrate(self, queries: list[str], top_prefilter: int, k: int = 4096, log=None) -> dict:
         """Recall@k of the proposal against exhaustive SW of the whole
@@ -657,8 +664,9 @@ class Proposer:
             call("sseqs_enc_to_fp8", ptr(q), q.numel(), FP8_SCALE, ptr(q8), st)
             groups.append((g0, g1, q8))
         ms.learned_groups = groups
-        if self._scores is None or self._scores.shape[1] != tab.n:
-            self._scores = torch.empty((self.group, tab.n), dtype=torch.float16, device=ms.dev)
+        if self._scores is None or self._scores.shape[1] != _ld(tab.n):
+            self._scores = torch.empty((self.group, _ld(tab.n)), dtype=torch.float16,
+                                       device=ms.dev)
         torch.cuda.synchronize(ms.dev)
         ms.query_embed_s = time.perf_counter() - t0
 
@@ -676,11 +684,12 @@ class Proposer:
         if getattr(ms, "learned_groups", None) is None:
             self.prepare(ms)
         for g0, g1, q8 in ms.learned_groups:
+            ld = self._scores.shape[1]
             call("sseqs_propose", ptr(tab.table), tab.n, ptr(q8), q8.shape[0], ptr(self._scores),
-                 tab.n, st)
+                 ld, st)
             for j in range(g1 - g0):
                 i = g0 + j
-                call("sseqs_topk", self._scores.data_ptr() + 2 * j * tab.n, _lib.F16, tab.n,
+                call("sseqs_topk", self._scores.data_ptr() + 2 * j * ld, _lib.F16, tab.n,
                      ms.kc, 0, ms.cand.data_ptr() + 8 * i * ms.kc, ptr(ms.sel_ws),
                      ms.sel_ws.numel(), st)
 

@@ -209,7 +209,7 @@ def test_table_and_proposal(orc, enc, dev):
     # proposal scores and candidates
     ms = MultiSearch(db, queries, top_prefilter=600, top_hits=50, proposer=Proposer(tab, group=16))
     ms.run()
-    sc = ms.proposer._scores[:3].float().cpu()
+    sc = ms.proposer._scores[:3, : db.n].float().cpu()
     qe = ms.proposer.table.enc.embed_strings(queries)
     qq = (FP8_SCALE * qe).cpu().to(torch.float8_e4m3fn).float()
     tt = tab.table.cpu().view(torch.float8_e4m3fn).float()
@@ -226,6 +226,18 @@ def test_table_and_proposal(orc, enc, dev):
         cand = ms.cand[i * 600:(i + 1) * 600].cpu().numpy()
         sw = orc.sw_affine_segments(orc.encode_query_sw(q), ascii, off[cand], ln[cand])
         assert np.array_equal(ms.cs[i * 600:(i + 1) * 600].cpu().numpy(), sw)
+    # a table whose length is not a multiple of 8 (nor of the kernel's 128
+    # rows), with the padded stride and with an unpadded one (scalar stores)
+    db2 = SeqDB.synthetic(1003, queries, seed=33, device=dev)
+    tab2 = EmbeddingTable(enc, db2)
+    s_pad = Proposer(tab2).scores_for(queries).float().cpu()
+    tt2 = tab2.table.cpu().view(torch.float8_e4m3fn).float()
+    assert torch.allclose(s_pad, (qq @ tt2.T).half().float(), rtol=2e-3, atol=2e-3)
+    q8 = torch.zeros((16, 256), dtype=torch.uint8, device=dev)
+    call("sseqs_enc_to_fp8", ptr(qe), qe.numel(), FP8_SCALE, ptr(q8), stream_of(dev))
+    raw = torch.full((16, 1003), float("nan"), dtype=torch.float16, device=dev)
+    call("sseqs_propose", ptr(tab2.table), 1003, ptr(q8), 16, ptr(raw), 1003, stream_of(dev))
+    assert torch.equal(raw[:3].float().cpu(), s_pad)
     # calibration against exhaustive SW (oracle scores of every sequence):
     # recall at K is the share of the exhaustive top k among the top K
     # proposals, and the gate decides between the learned and plain selection
