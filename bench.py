@@ -208,7 +208,7 @@ def main() -> int:
     ap.add_argument("--no-learned", action="store_true", help="skip the learned prefilter")
     ap.add_argument("--learned-only", action="store_true",
                     help="run only the learned prefilter measurement")
-    ap.add_argument("--recall-queries", type=int, default=4,
+    ap.add_argument("--recall-queries", type=int, default=16,
                     help="batched queries whose Recall@4096 is measured against exhaustive SW")
     ap.add_argument("--recall-timed", type=int, default=2,
                     help="of those, how many get the plain cascade timed at its 0.99 point")
@@ -665,8 +665,12 @@ def main() -> int:
                                                      1),
                "recall_queries": len(rec),
                "learned_recall_min": min(r["learned_recall_at_top_prefilter"] for r in rec),
+               "learned_recall_p10": round(float(np.quantile(
+                   [r["learned_recall_at_top_prefilter"] for r in rec], 0.1)), 4),
                "learned_recall_mean": round(float(np.mean(
                    [r["learned_recall_at_top_prefilter"] for r in rec])), 4),
+               "learned_k99_median": int(np.median([r["learned_top_prefilter_for_0.99"]
+                                                    for r in rec])),
                "plain_recall_mean": round(float(np.mean(
                    [r["plain_recall_at_top_prefilter"] for r in rec])), 4),
                "learned_k99_max": max(r["learned_top_prefilter_for_0.99"] for r in rec),
