@@ -500,7 +500,7 @@ def main() -> int:
         log(json.dumps(res))
         return res
 
-    def measure_batched(queries: list[str], proposer=None):
+    def measure_batched(queries: list[str], proposer=None, top_prefilter=None, graph=True):
         """The batched workload: every query's whole-database cascade
         (prefilter, top --top-prefilter, affine SW 11/1, top --top-hits,
         backtrack) in one sweep captured as a HIP graph; one replay timed
@@ -509,7 +509,8 @@ def main() -> int:
         from sseqs_amd.multi import MultiSearch
         from sseqs_amd.search import sw_affine_ids
         t0 = time.perf_counter()
-        ms = MultiSearch(db, queries, args.top_prefilter, args.top_hits, proposer=proposer)
+        kc = args.top_prefilter if top_prefilter is None else top_prefilter
+        ms = MultiSearch(db, queries, kc, args.top_hits, proposer=proposer)
         # the eager sweep (warm-up, with events between its stages), then
         # the capture and one timed replay of the graph
         ev = [torch.cuda.Event(enable_timing=True) for _ in range(5)]
@@ -524,29 +525,35 @@ def main() -> int:
         ev[4].record()
         torch.cuda.synchronize()
         st = [ev[i].elapsed_time(ev[i + 1]) for i in range(4)]
-        g = torch.cuda.CUDAGraph()
-        with torch.cuda.graph(g):
-            ms.run()
-        ms.graph = g
-        torch.cuda.synchronize()
+        if graph:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                ms.run()
+            ms.graph = g
+            torch.cuda.synchronize()
         setup = time.perf_counter() - t0
-        log(f"batched sweep of {len(queries)} queries set up and captured in {setup:.1f} s")
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        ms.replay()
-        e1.record()
-        torch.cuda.synchronize()
-        graph_ms = e0.elapsed_time(e1)
+        log(f"batched sweep of {len(queries)} queries set up{' and captured' if graph else ''} "
+            f"in {setup:.1f} s")
+        if graph:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            ms.replay()
+            e1.record()
+            torch.cuda.synchronize()
+            graph_ms = e0.elapsed_time(e1)
+        else:  # the eager sweep's stages
+            graph_ms = float(sum(st))
         lens = [len(t) for t in queries]
         cand_res = int(db.seq_len[ms.cand].to(torch.int64).sum().item())
-        cells = sum(int(ms.q_len[i]) * int(db.seq_len[ms.cand[i * ms.kc:(i + 1) * ms.kc]]
+        cells = sum(int(ms.q_len[i]) * int(db.seq_len[ms.cand[ms.c_off_h[i]:ms.c_off_h[i + 1]]]
                                               .to(torch.int64).sum().item())
                     for i in range(ms.nq))
         # scores bit-equal to single-query launches (sampled queries)
         same = True
         for i in range(0, ms.nq, max(1, ms.nq // 8)):
-            one = sw_affine_ids(db, ms.qs[i], ms.cand[i * ms.kc:(i + 1) * ms.kc])
-            same &= bool(torch.equal(one, ms.cs[i * ms.kc:(i + 1) * ms.kc]))
+            c = slice(int(ms.c_off_h[i]), int(ms.c_off_h[i + 1]))
+            one = sw_affine_ids(db, ms.qs[i], ms.cand[c])
+            same &= bool(torch.equal(one, ms.cs[c]))
         pf_flops = 0.0
         for q in ms.qs:
             for N, b, e in db.length_ranges(q.length):
@@ -557,8 +564,12 @@ def main() -> int:
         sec = graph_ms / 1e3
         res = {"queries": len(queries), "q_len_min": min(lens), "q_len_max": max(lens),
                "q_len_mean": round(float(np.mean(lens)), 1),
-               "top_prefilter": ms.kc, "top_hits": ms.kh,
-               "sweep_ms_graph_replay": round(graph_ms, 1),
+               "top_prefilter": (ms.kc if np.all(ms.kc_q == ms.kc)
+                                 else {"per_query_mean": round(float(ms.kc_q.mean())),
+                                       "min": int(ms.kc_q.min()), "max": int(ms.kc_q.max())}),
+               "top_hits": ms.kh,
+               "sweep_ms": round(graph_ms, 1),
+               "timed": "HIP-graph replay" if graph else "eager sweep",
                "ms_per_query": round(graph_ms / len(queries), 2),
                "queries_per_s": round(len(queries) / sec, 2),
                "search_gcups": round(sum(lens) * db.residues / sec / 1e9, 1),
@@ -605,6 +616,14 @@ def main() -> int:
         k = min(4096, kh)
         sample = np.linspace(0, len(queries) - 1, args.recall_queries).round().astype(int)
         sample = np.unique(sample)
+        # the calibrated operating point: measured on queries halfway between
+        # the sample's (held out from it), applied to the sample
+        half = max(1, (len(queries) - 1) // max(1, 2 * (args.recall_queries - 1)))
+        cal_idx = np.setdiff1d(np.unique(np.clip(sample + half, 0, len(queries) - 1)), sample)
+        cal = prop.calibrate([queries[i] for i in cal_idx], args.top_prefilter, k, log=log)
+        log(f"calibration on {len(cal_idx)} held-out queries: recall at top "
+            f"{args.top_prefilter} min {cal['recall_min']}, mean {round(cal['recall_mean'], 4)}; "
+            f"candidates by query length {cal['k_by_len']}")
         rec = []
         grid = (100_000, 300_000, 1_000_000, 3_000_000, 10_000_000, 30_000_000, 100_000_000)
         for i in sample:
@@ -622,23 +641,27 @@ def main() -> int:
             gh = global_search(db, q, args.top_prefilter, k, scores_buf=scores_buf)
             r["plain_recall_at_top_prefilter"] = round(
                 min(k, int((gh.scores >= kth).sum().item())) / k, 4)
+            Kc = int(prop.top_prefilter_for([qtext])[0])
             for name, sc in (("plain", gh.prefilter), ("learned", prop.scores_for([qtext])[0])):
-                curve, K99 = recall_curve(sc, good, k, grid)
-                r[f"{name}_recall_curve"] = {str(K): v for K, v in curve.items()}
+                curve, K99 = recall_curve(sc, good, k, grid + ((Kc,) if name == "learned" else ()))
+                if name == "learned":
+                    r["learned_calibrated_top_prefilter"] = Kc
+                    r["learned_recall_at_calibrated"] = curve[Kc]
+                r[f"{name}_recall_curve"] = {str(K): v for K, v in curve.items() if K in grid}
                 r[f"{name}_top_prefilter_for_0.99"] = K99
-            # each cascade at its own 0.99 operating point, timed (for the
-            # first --recall-timed queries of the sample)
+            # the learned cascade at its calibrated operating point and the
+            # plain one at its 0.99 point, timed (the first --recall-timed
+            # queries of the sample)
             if len(rec) < args.recall_timed:
                 from sseqs_amd.multi import MultiSearch
-                KL = r["learned_top_prefilter_for_0.99"]
-                m1 = MultiSearch(db, [qtext], KL, k, proposer=prop, backtrack=True)
+                m1 = MultiSearch(db, [qtext], Kc, k, proposer=prop, backtrack=True)
                 m1.run()
                 torch.cuda.synchronize()
                 t = time.perf_counter()
                 m1.run()
                 torch.cuda.synchronize()
-                r["learned_ms_at_0.99"] = round(1e3 * (time.perf_counter() - t), 1)
-                r["learned_recall_measured_at_0.99"] = round(
+                r["learned_ms_at_calibrated"] = round(1e3 * (time.perf_counter() - t), 1)
+                r["learned_recall_measured_at_calibrated"] = round(
                     min(k, int((m1.hit_sc >= kth).sum().item())) / k, 4)
                 del m1
                 K99 = r["plain_top_prefilter_for_0.99"]
@@ -659,10 +682,16 @@ def main() -> int:
                "plain_ms_per_query_at_0.99": round(float(np.mean([r["plain_ms_at_0.99"]
                                                                   for r in rec
                                                                   if "plain_ms_at_0.99" in r])), 1),
-               "learned_ms_per_query_at_0.99": round(float(np.mean([r["learned_ms_at_0.99"]
-                                                                    for r in rec
-                                                                    if "learned_ms_at_0.99" in r])),
-                                                     1),
+               "learned_ms_per_query_at_calibrated": round(float(np.mean(
+                   [r["learned_ms_at_calibrated"] for r in rec
+                    if "learned_ms_at_calibrated" in r])), 1),
+               "calibration": {k_: v for k_, v in cal.items() if k_ != "per_query"},
+               "learned_recall_at_calibrated_min": min(r["learned_recall_at_calibrated"]
+                                                       for r in rec),
+               "learned_recall_at_calibrated_mean": round(float(np.mean(
+                   [r["learned_recall_at_calibrated"] for r in rec])), 4),
+               "learned_calibrated_top_prefilter_mean": round(float(np.mean(
+                   [r["learned_calibrated_top_prefilter"] for r in rec]))),
                "recall_queries": len(rec),
                "learned_recall_min": min(r["learned_recall_at_top_prefilter"] for r in rec),
                "learned_recall_p10": round(float(np.quantile(
@@ -679,21 +708,31 @@ def main() -> int:
         if plain_batched:
             out["plain_batched_ms_per_query_top_prefilter"] = plain_batched["ms_per_query"]
         out["speedup_vs_plain_at_0.99"] = round(out["plain_ms_per_query_at_0.99"] /
-                                                out["learned_ms_per_query_at_0.99"], 1)
+                                                out["learned_ms_per_query_at_calibrated"], 1)
         log(json.dumps({k_: v for k_, v in out.items() if k_ not in ("sweep", "recall_sample")}))
         del ms
         torch.cuda.empty_cache()
+        # the whole batch at the calibrated operating point
+        got = side("batched sweep at the calibrated point", lambda: measure_batched(
+            queries, proposer=prop, top_prefilter=prop.top_prefilter_for(queries), graph=False))
+        if got is not None:
+            out["sweep_calibrated"] = got[0]
+            del got
+            torch.cuda.empty_cache()
         if args.serve_requests > 0:
             # serving: the learned prefilter at the configured candidate count
-            # and both prefilters at their median 0.99-recall operating point
-            k_l = int(np.median([r["learned_top_prefilter_for_0.99"] for r in rec]))
+            # and at its calibrated operating point, the plain one at its
+            # median 0.99-recall point
             k_p = int(np.median([r["plain_top_prefilter_for_0.99"] for r in rec]))
             out["serving"] = [
                 side("serving learned", lambda: measure_serving(
                     args.serve_rate_high, args.serve_requests, args.top_prefilter, prop,
                     "learned")),
-                side("serving learned 0.99", lambda: measure_serving(
-                    args.serve_rate_low, args.serve_plain_requests, k_l, prop, "learned")),
+                side("serving learned calibrated", lambda: measure_serving(
+                    args.serve_rate_high, args.serve_requests, "calibrated", prop, "learned")),
+                side("serving learned calibrated", lambda: measure_serving(
+                    args.serve_rate_low, args.serve_requests // 4, "calibrated", prop,
+                    "learned")),
                 side("serving plain 0.99", lambda: measure_serving(
                     args.serve_rate_low, args.serve_plain_requests, k_p, None, "plain")),
             ]
@@ -732,6 +771,8 @@ def main() -> int:
             bs = list(eng.stats.batch_sizes)
         res = {"prefilter": label, "offered_req_per_s": rate, "requests": n_req,
                "top_prefilter": top_prefilter,
+               "mean_top_prefilter": (round(float(np.mean(proposer.top_prefilter_for(qs))))
+                                      if top_prefilter == "calibrated" else top_prefilter),
                "achieved_req_per_s": round(n_req / (t1 - t0), 2),
                "p50_ms": round(float(np.percentile(lat, 50)), 1),
                "p99_ms": round(float(np.percentile(lat, 99)), 1),

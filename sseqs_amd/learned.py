@@ -336,7 +336,8 @@ def sw_pairs(db, queries: list[str], cand: list[np.ndarray]) -> np.ndarray:
     ms = MultiSearch(db, queries, top_prefilter=1, top_hits=1, backtrack=False, tuning={})
     counts = np.array([c.size for c in cand], dtype=np.int64)
     c_all = np.concatenate(cand).astype(np.int64)
-    ms.kc = int(counts.max())
+    ms.kc, ms.kc_q = int(counts.max()), counts
+    ms.c_off_h = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
     ms.cand = torch.from_numpy(c_all).to(db.device)
     ms.cs = torch.empty(c_all.size, dtype=torch.int32, device=db.device)
     ms.c_off = torch.from_numpy(np.concatenate([[0], np.cumsum(counts)])).to(db.device)
@@ -600,6 +601,7 @@ class Proposer:
         self.gate = gate
         self.use_learned = True
         self.calibration: dict | None = None
+        self.k_by_len: list[tuple[int, int, int]] | None = None
         self._scores = None
 
     def scores_for(self, queries: list[str]) -> torch.Tensor:
@@ -620,13 +622,20 @@ class Proposer:
         call("sseqs_propose", ptr(tab.table), tab.n, ptr(q8), Gp, ptr(out), ld, st)
         return out[: len(queries), : tab.n]
 
-    def calibrate(self, queries: list[str], top_prefilter: int, k: int = 4096, log=None) -> dict:
+    # query-length classes of the calibrated operating point
+    LEN_EDGES = (0, 100, 200, 350, 500, 700, 2049)
+
+    def calibrate(self, queries: list[str], top_prefilter: int, k: int = 4096, log=None,
+                  margin: float = 1.25) -> dict:
         """Recall@k of the proposal against exhaustive SW of the whole
         database for sample queries: per query, the recall among the top
         ``top_prefilter`` proposals and the candidate count that reaches the
         gate.  ``use_learned`` becomes whether every sample query reaches
         the gate at ``top_prefilter``; otherwise the plain prefilter
-        selects."""
+        selects.  The calibrated operating point (``top_prefilter_for``)
+        gives a query of each length class ``margin`` times the largest
+        count that class's sample queries needed (every sample query's when
+        the class has none), at least k."""
         db = self.table.db
         per = []
         for q in queries:
@@ -640,11 +649,32 @@ class Proposer:
                 log(f"[calibrate] {json.dumps(per[-1])}")
         rec = [p["recall"] for p in per]
         self.use_learned = bool(per) and min(rec) >= self.gate
+        kg = np.array([p["top_prefilter_for_gate"] for p in per], np.float64)
+        ql = np.array([p["q_len"] for p in per])
+        edges = self.LEN_EDGES
+        self.k_by_len = []
+        for lo, hi in zip(edges, edges[1:]):
+            inside = kg[(ql >= lo) & (ql < hi)]
+            top = inside.max() if inside.size else (kg.max() if kg.size else db.n)
+            self.k_by_len.append((lo, hi, int(min(db.n, max(k, math.ceil(margin * top))))))
         self.calibration = {"queries": len(per), "top_prefilter": top_prefilter, "k": k,
                             "gate": self.gate, "recall_min": min(rec, default=None),
                             "recall_mean": float(np.mean(rec)) if rec else None,
-                            "use_learned": self.use_learned, "per_query": per}
+                            "use_learned": self.use_learned, "k_by_len": self.k_by_len,
+                            "per_query": per}
         return self.calibration
+
+    def top_prefilter_for(self, queries: list[str]) -> np.ndarray:
+        """Candidate counts of the calibrated operating point (int64, one
+        per query); ``calibrate`` first."""
+        if self.k_by_len is None:
+            raise RuntimeError("calibrate() first")
+        out = np.empty(len(queries), np.int64)
+        for i, q in enumerate(queries):
+            L = len(q)
+            out[i] = next((K for lo, hi, K in self.k_by_len if lo <= L < hi),
+                          self.k_by_len[-1][2])
+        return out
 
     def prepare(self, ms) -> None:
         """Set-up of a sweep (before any capture): the queries' fp8
@@ -690,8 +720,8 @@ class Proposer:
             for j in range(g1 - g0):
                 i = g0 + j
                 call("sseqs_topk", self._scores.data_ptr() + 2 * j * ld, _lib.F16, tab.n,
-                     ms.kc, 0, ms.cand.data_ptr() + 8 * i * ms.kc, ptr(ms.sel_ws),
-                     ms.sel_ws.numel(), st)
+                     int(ms.kc_q[i]), 0, ms.cand.data_ptr() + 8 * int(ms.c_off_h[i]),
+                     ptr(ms.sel_ws), ms.sel_ws.numel(), st)
 
 
 def recall_curve(scores: torch.Tensor, good: torch.Tensor, k: int, grid=(),

@@ -122,10 +122,19 @@ class MultiSearch:
             if L == 0 or L > 2048:
                 raise ValueError("every query needs 1..2048 residues")
         self.nq = len(self.qs)
-        self.kc = min(int(top_prefilter), db.n)
-        self.kh = min(int(top_hits), self.kc)
+        # candidates per query: one count for all, or one per query (then
+        # the lists are laid out back to back, query i's at c_off[i])
+        kq = np.asarray(top_prefilter, dtype=np.int64).reshape(-1)
+        if kq.size == 1:
+            kq = np.full(self.nq, int(kq[0]), np.int64)
+        if kq.size != self.nq or np.any(kq < 1):
+            raise ValueError("top_prefilter: a positive count, or one per query")
+        self.kc_q = np.minimum(kq, db.n)
+        self.kc = int(self.kc_q.max())
+        self.kh = min(int(top_hits), int(self.kc_q.min()))
         if self.kh > 4096:
             raise ValueError("top_hits is limited to 4096 per query")
+        self.c_off_h = np.concatenate([[0], np.cumsum(self.kc_q)]).astype(np.int64)
         self.go, self.ge = int(gap_open), int(gap_extend)
         # candidate selection: the plain prefilter's top kc, or a proposer's
         # (an object with select(ms) filling ms.cand, e.g. learned.Proposer)
@@ -135,23 +144,23 @@ class MultiSearch:
         lib = _lib.load()
         self.scores = torch.empty(db.n, dtype=torch.float32, device=dev)
         self.sel_ws = _lib.workspace(lib.sseqs_topk_ws_bytes(db.n), dev)
-        self.cand = torch.empty(self.nq * self.kc, dtype=torch.int64, device=dev)
-        self.cs = torch.empty(self.nq * self.kc, dtype=torch.int32, device=dev)
+        self.cand = torch.empty(int(self.c_off_h[-1]), dtype=torch.int64, device=dev)
+        self.cs = torch.empty(int(self.c_off_h[-1]), dtype=torch.int32, device=dev)
         codes = [q.codes.cpu().numpy() for q in self.qs]
         q_off = np.concatenate([[0], np.cumsum([c.size for c in codes])]).astype(np.int64)
         self.qcodes = torch.from_numpy(np.concatenate(codes)).to(dev)
         self.q_off = torch.from_numpy(q_off).to(dev)
-        self.c_off = torch.arange(self.nq + 1, dtype=torch.int64, device=dev) * self.kc
+        self.c_off = torch.from_numpy(self.c_off_h).to(dev)
         self.q_len = np.diff(q_off)
         self.long_queries = [i for i in range(self.nq) if self.q_len[i] > 1024]
         self.set_sw_tuning(self.tuning)
         # hits: one segmented select over all queries
-        self.seg_begin = np.arange(self.nq, dtype=np.int64) * self.kc
+        self.seg_begin = self.c_off_h[:-1].copy()
         self.hit_pos = torch.empty(self.nq * self.kh, dtype=torch.int64, device=dev)
         self.meta64 = torch.from_numpy(np.concatenate(
             [self.seg_begin, np.arange(self.nq, dtype=np.int64) * self.kh])).to(dev)
         self.meta32 = torch.from_numpy(np.concatenate(
-            [np.full(self.nq, self.kc, np.int32), np.full(self.nq, self.kh, np.int32)])).to(dev)
+            [self.kc_q.astype(np.int32), np.full(self.nq, self.kh, np.int32)])).to(dev)
         self.hit_ids = torch.empty(self.nq * self.kh, dtype=torch.int64, device=dev)
         self.hit_sc = torch.empty(self.nq * self.kh, dtype=torch.int32, device=dev)
         self.backtrack = bool(backtrack)
@@ -179,9 +188,9 @@ class MultiSearch:
             qi = [i for i in range(self.nq) if lens[i] <= 1024 and (lens[i] + 31) // 32 == tpt]
             prm = sw_params(self.tuning, tpt)
             item = int(prm["item"])
-            per = -(-self.kc // item)
+            per = [-(-int(self.kc_q[i]) // item) for i in qi]
             iq = np.repeat(np.array(qi, np.int32), per)
-            ij = np.tile(np.arange(per, dtype=np.int32), len(qi))
+            ij = np.concatenate([np.arange(p, dtype=np.int32) for p in per])
             grid = int(min(N_CU * int(prm["blocks_per_cu"]), iq.size))
             self.buckets.append(dict(
                 tpt=tpt, item=item, grid=grid, n=int(iq.size),
@@ -201,9 +210,9 @@ class MultiSearch:
                     call("sseqs_db_prefilter", ptr(self.db.packed), ptr(self.db.seq_off),
                          ptr(self.db.seq_len), b, e, ptr(q.qrev), q.length, ptr(q.lut_a),
                          ptr(q.lut_b_flat), 21, q.k, N, N, self.scores.data_ptr() + 4 * b, st)
-            call("sseqs_topk", ptr(self.scores), _lib.F32, self.db.n, self.kc, 0,
-                 self.cand.data_ptr() + 8 * i * self.kc, ptr(self.sel_ws), self.sel_ws.numel(),
-                 st)
+            call("sseqs_topk", ptr(self.scores), _lib.F32, self.db.n, int(self.kc_q[i]), 0,
+                 self.cand.data_ptr() + 8 * int(self.c_off_h[i]), ptr(self.sel_ws),
+                 self.sel_ws.numel(), st)
 
     def _sw(self) -> None:
         st = stream_of(self.dev)
@@ -215,10 +224,10 @@ class MultiSearch:
                  self.go, self.ge, ptr(self.cs), ptr(b["counter"]), st)
         for i in self.long_queries:  # past 1024 residues: the 32-bit kernel
             q = self.qs[i]
+            c0 = int(self.c_off_h[i])
             call("sseqs_db_sw_affine", ptr(db.packed), ptr(db.seq_off), ptr(db.seq_len),
-                 self.cand.data_ptr() + 8 * i * self.kc, self.kc, ptr(q.codes),
-                 int(q.codes.numel()), self.go, self.ge, self.cs.data_ptr() + 4 * i * self.kc,
-                 st)
+                 self.cand.data_ptr() + 8 * c0, int(self.kc_q[i]), ptr(q.codes),
+                 int(q.codes.numel()), self.go, self.ge, self.cs.data_ptr() + 4 * c0, st)
 
     def _hits(self) -> None:
         st = stream_of(self.dev)
@@ -268,7 +277,7 @@ class MultiSearch:
     def results(self) -> list[QueryHits]:
         out = []
         for i, t in enumerate(self.texts):
-            s = slice(i * self.kc, (i + 1) * self.kc)
+            s = slice(int(self.c_off_h[i]), int(self.c_off_h[i + 1]))
             h = slice(i * self.kh, (i + 1) * self.kh)
             al = None
             if self.backtrack:

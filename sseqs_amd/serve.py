@@ -86,7 +86,15 @@ class SearchEngine:
                  max_batch: int = 64, hbm_budget: int | None = None, in_flight: int = 2,
                  proposer=None, hit_width: int = 500, max_wait_s: float = 0.0):
         self.db = db
-        self.kc, self.kh = int(top_prefilter), int(top_hits)
+        # candidates per query: a count, or "calibrated" for the proposer's
+        # operating point by query length (Proposer.calibrate)
+        if top_prefilter == "calibrated":
+            if proposer is None or getattr(proposer, "k_by_len", None) is None:
+                raise ValueError("top_prefilter='calibrated' needs a calibrated proposer")
+            self.kc = "calibrated"
+        else:
+            self.kc = int(top_prefilter)
+        self.kh = int(top_hits)
         self.max_batch = int(max_batch)
         self.in_flight = max(1, int(in_flight))
         self.proposer = proposer
@@ -143,14 +151,20 @@ class SearchEngine:
         while self._q and len(batch) < self.max_batch:
             r = self._q[0]
             lq = max(lq_max, len(r.query))
-            need = (sweep_bytes(self.db, lq, self.kh, self.hit_width) + used +
-                    query_bytes(self.db, len(r.query), self.kc, self.kh, self.hit_width))
+            qb = query_bytes(self.db, len(r.query), self._kc(r.query), self.kh, self.hit_width)
+            need = sweep_bytes(self.db, lq, self.kh, self.hit_width) + used + qb
             if batch and need > self.budget:
                 break
             batch.append(self._q.popleft())
-            used += query_bytes(self.db, len(r.query), self.kc, self.kh, self.hit_width)
+            used += qb
             lq_max = lq
         return batch
+
+    def _kc(self, query: str) -> int:
+        """Candidates passed to SW for one query."""
+        if self.kc == "calibrated":
+            return int(self.proposer.top_prefilter_for([query])[0])
+        return self.kc
 
     # ------------------------------------------------------------- loop --
     def _run(self) -> None:
@@ -174,7 +188,8 @@ class SearchEngine:
             for r in batch:
                 r.t_start = t0
             try:
-                ms = MultiSearch(self.db, [r.query for r in batch], self.kc, self.kh,
+                ms = MultiSearch(self.db, [r.query for r in batch],
+                                 [self._kc(r.query) for r in batch], self.kh,
                                  hit_width=self.hit_width, proposer=self.proposer)
                 s1, s2 = self._streams
                 # the sweep's inputs were copied to the device on this stream

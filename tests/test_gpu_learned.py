@@ -252,6 +252,14 @@ def test_table_and_proposal(orc, enc, dev):
         assert cal["per_query"][i]["kth_exhaustive_score"] == kth
         assert abs(cal["per_query"][i]["recall"] - round(rec, 4)) < 1e-9, i
     assert prop.use_learned == (min(p["recall"] for p in cal["per_query"]) >= 0.99)
+    # the calibrated operating point gives each calibration query at least
+    # the count it needed for 0.99
+    Ks = prop.top_prefilter_for(queries)
+    for i, pq in enumerate(cal["per_query"]):
+        assert Ks[i] >= min(db.n, max(k, pq["top_prefilter_for_gate"])), i
+    from sseqs_amd.serve import SearchEngine
+    with pytest.raises(ValueError):
+        SearchEngine(db, top_prefilter="calibrated")
     cal = prop.calibrate(queries, top_prefilter=db.n, k=k)
     assert prop.use_learned and all(p["recall"] == 1.0 for p in cal["per_query"])
     prop.calibrate(queries, top_prefilter=1, k=k)

@@ -100,3 +100,39 @@ def test_multi_graph_replay(mworld):
             for x, y in zip(a[:4], b[:4]):
                 assert torch.equal(x, y), rep
             assert a[4] == b[4], rep
+
+
+def test_multi_per_query_candidate_counts(mworld):
+    """One candidate count per query (lists laid out back to back): every
+    query's candidates, scores and hits equal its single-query run with its
+    own count, with the plain prefilter and after a graph capture."""
+    from sseqs_amd.multi import MultiSearch
+    from sseqs_amd.query import prepare
+    from sseqs_amd.search import global_search
+    announce("batched sweep with per-query candidate counts")
+    db = mworld["db"]
+    qs = mworld["queries"][:7]
+    kc = [3000, 500, 4096, 777, 12000, 401, 2048]
+    ms = MultiSearch(db, qs, kc, TOP_HITS)
+    assert ms.cand.numel() == sum(kc) and ms.kh == TOP_HITS
+    ms.run()
+    torch.cuda.synchronize()
+    want = _snapshot(ms)
+    for i, (q, r) in enumerate(zip(qs, ms.results())):
+        gh = global_search(db, prepare(q), kc[i], TOP_HITS)
+        assert r.candidates.numel() == kc[i], i
+        assert torch.equal(r.candidates, gh.candidates), i
+        assert torch.equal(r.cand_scores, gh.cand_scores), i
+        assert torch.equal(r.ids, gh.ids) and torch.equal(r.scores, gh.scores), i
+    ms.capture()
+    for t in (ms.cand, ms.cs, ms.hit_ids, ms.hit_sc):
+        t.fill_(-3)
+    torch.cuda.synchronize()
+    ms.replay()
+    torch.cuda.synchronize()
+    for a, b in zip(_snapshot(ms), want):
+        for x, y in zip(a[:4], b[:4]):
+            assert torch.equal(x, y)
+        assert a[4] == b[4]
+    with pytest.raises(ValueError):
+        MultiSearch(db, qs, kc[:3], TOP_HITS)
