@@ -724,10 +724,14 @@ def main() -> int:
             # and at its calibrated operating point, the plain one at its
             # median 0.99-recall point
             k_p = int(np.median([r["plain_top_prefilter_for_0.99"] for r in rec]))
+            prop.enforce_gate = False  # the proposal at 100 k for every query
             out["serving"] = [
                 side("serving learned", lambda: measure_serving(
                     args.serve_rate_high, args.serve_requests, args.top_prefilter, prop,
                     "learned")),
+            ]
+            prop.enforce_gate = True
+            out["serving"] += [
                 side("serving learned calibrated", lambda: measure_serving(
                     args.serve_rate_high, args.serve_requests, "calibrated", prop, "learned")),
                 side("serving learned calibrated", lambda: measure_serving(

@@ -22,8 +22,9 @@ of queries is embedded the same way and the proposal GEMM
 query's top ``top_prefilter`` go to exact affine SW, which stays the judge.
 ``Proposer`` plugs this into ``multi.MultiSearch`` (and so into the serving
 engine); its ``calibrate`` measures Recall@4096 against exhaustive SW on
-sample queries, and when that falls below the gate (0.99) the proposer
-falls back to the plain prefilter.
+sample queries and sets the candidate count per query length that reaches
+the gate (0.99); a query given fewer candidates than that falls back to the
+plain prefilter.
 
 There is no counterpart in the reference, which only has the FFT prefilter
 (sseqs/msa.py:270-310); the candidates this proposes replace that
@@ -592,8 +593,9 @@ def _ld(n: int) -> int:
 class Proposer:
     """Candidate selection for multi.MultiSearch from the embedding table:
     per query, the top kc sequences by e_q . e_t (FP8 MFMA), in ascending
-    index order.  ``use_learned`` is False after a failed calibration, and
-    then the plain prefilter selects."""
+    index order.  After ``calibrate``, a query whose candidate count does
+    not reach the recall gate (``meets_gate``) is selected by the plain
+    prefilter instead."""
 
     def __init__(self, table: EmbeddingTable, group: int = 64, gate: float = 0.99):
         self.table = table
@@ -602,6 +604,7 @@ class Proposer:
         self.use_learned = True
         self.calibration: dict | None = None
         self.k_by_len: list[tuple[int, int, int]] | None = None
+        self.enforce_gate = True
         self._scores = None
 
     def scores_for(self, queries: list[str]) -> torch.Tensor:
@@ -631,11 +634,11 @@ class Proposer:
         database for sample queries: per query, the recall among the top
         ``top_prefilter`` proposals and the candidate count that reaches the
         gate.  ``use_learned`` becomes whether every sample query reaches
-        the gate at ``top_prefilter``; otherwise the plain prefilter
-        selects.  The calibrated operating point (``top_prefilter_for``)
+        the gate at ``top_prefilter``.  The calibrated operating point (``top_prefilter_for``)
         gives a query of each length class ``margin`` times the largest
         count that class's sample queries needed (every sample query's when
-        the class has none), at least k."""
+        the class has none), at least k.  A query whose candidate count
+        meets neither is selected by the plain prefilter (``meets_gate``)."""
         db = self.table.db
         per = []
         for q in queries:
@@ -700,28 +703,46 @@ class Proposer:
         torch.cuda.synchronize(ms.dev)
         ms.query_embed_s = time.perf_counter() - t0
 
+    def meets_gate(self, texts: list[str], counts) -> np.ndarray:
+        """Per query, whether the learned proposal reaches the recall gate
+        with ``counts`` candidates, as calibrated: the count reaches the
+        query's calibrated operating point, or the gate held for every
+        calibration query at a count no larger.  Before any calibration (or
+        with ``enforce_gate`` off) every query takes the learned proposal."""
+        counts = np.asarray(counts, np.int64).reshape(-1)
+        if not self.enforce_gate or self.calibration is None:
+            return np.ones(len(texts), bool)
+        ok = np.zeros(len(texts), bool)
+        if self.k_by_len is not None:
+            ok |= counts >= self.top_prefilter_for(texts)
+        if self.use_learned:
+            ok |= counts >= self.calibration["top_prefilter"]
+        return ok
+
     def select(self, ms) -> None:
-        if not self.use_learned:
-            saved = ms.proposer
-            ms.proposer = None
-            try:
-                ms._prefilter_select()
-            finally:
-                ms.proposer = saved
+        """Candidates of every query of a sweep: the learned proposal's top
+        kc_q[i] where that meets the gate, the plain prefilter's elsewhere."""
+        ok = self.meets_gate(ms.texts, ms.kc_q)
+        for i in np.nonzero(~ok)[0]:
+            ms.plain_select(int(i))
+        if not ok.any():
             return
         tab = self.table
         st = stream_of(ms.dev)
         if getattr(ms, "learned_groups", None) is None:
             self.prepare(ms)
         for g0, g1, q8 in ms.learned_groups:
+            if not ok[g0:g1].any():
+                continue
             ld = self._scores.shape[1]
             call("sseqs_propose", ptr(tab.table), tab.n, ptr(q8), q8.shape[0], ptr(self._scores),
                  ld, st)
             for j in range(g1 - g0):
                 i = g0 + j
-                call("sseqs_topk", self._scores.data_ptr() + 2 * j * ld, _lib.F16, tab.n,
-                     int(ms.kc_q[i]), 0, ms.cand.data_ptr() + 8 * int(ms.c_off_h[i]),
-                     ptr(ms.sel_ws), ms.sel_ws.numel(), st)
+                if ok[i]:
+                    call("sseqs_topk", self._scores.data_ptr() + 2 * j * ld, _lib.F16, tab.n,
+                         int(ms.kc_q[i]), 0, ms.cand.data_ptr() + 8 * int(ms.c_off_h[i]),
+                         ptr(ms.sel_ws), ms.sel_ws.numel(), st)
 
 
 def recall_curve(scores: torch.Tensor, good: torch.Tensor, k: int, grid=(),

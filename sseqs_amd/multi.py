@@ -202,17 +202,22 @@ class MultiSearch:
         if self.proposer is not None:
             self.proposer.select(self)
             return
+        for i in range(self.nq):
+            self.plain_select(i)
+
+    def plain_select(self, i: int) -> None:
+        """Query i's candidates from the plain prefilter: its scores of every
+        sequence, then their top kc_q[i]."""
         st = stream_of(self.dev)
-        for i, q in enumerate(self.qs):
-            ranges = self.db.length_ranges(q.length)
-            for N, b, e in ranges:
-                with prefilter_env(self.tuning, N):
-                    call("sseqs_db_prefilter", ptr(self.db.packed), ptr(self.db.seq_off),
-                         ptr(self.db.seq_len), b, e, ptr(q.qrev), q.length, ptr(q.lut_a),
-                         ptr(q.lut_b_flat), 21, q.k, N, N, self.scores.data_ptr() + 4 * b, st)
-            call("sseqs_topk", ptr(self.scores), _lib.F32, self.db.n, int(self.kc_q[i]), 0,
-                 self.cand.data_ptr() + 8 * int(self.c_off_h[i]), ptr(self.sel_ws),
-                 self.sel_ws.numel(), st)
+        q = self.qs[i]
+        for N, b, e in self.db.length_ranges(q.length):
+            with prefilter_env(self.tuning, N):
+                call("sseqs_db_prefilter", ptr(self.db.packed), ptr(self.db.seq_off),
+                     ptr(self.db.seq_len), b, e, ptr(q.qrev), q.length, ptr(q.lut_a),
+                     ptr(q.lut_b_flat), 21, q.k, N, N, self.scores.data_ptr() + 4 * b, st)
+        call("sseqs_topk", ptr(self.scores), _lib.F32, self.db.n, int(self.kc_q[i]), 0,
+             self.cand.data_ptr() + 8 * int(self.c_off_h[i]), ptr(self.sel_ws),
+             self.sel_ws.numel(), st)
 
     def _sw(self) -> None:
         st = stream_of(self.dev)

@@ -262,9 +262,24 @@ def test_table_and_proposal(orc, enc, dev):
         SearchEngine(db, top_prefilter="calibrated")
     cal = prop.calibrate(queries, top_prefilter=db.n, k=k)
     assert prop.use_learned and all(p["recall"] == 1.0 for p in cal["per_query"])
-    prop.calibrate(queries, top_prefilter=1, k=k)
+    cal1 = prop.calibrate(queries, top_prefilter=1, k=k)
     assert not prop.use_learned
-    plain = MultiSearch(db, queries, top_prefilter=600, top_hits=50)
+    # counts below each query's calibrated point fall back to the plain
+    # prefilter, query by query; counts at it take the learned proposal
+    Kq = prop.top_prefilter_for(queries)
+    assert np.array_equal(prop.meets_gate(queries, Kq), np.ones(3, bool))
+    kc = np.array([600, int(Kq[1]), 600])
+    want_ok = kc >= Kq
+    assert np.array_equal(prop.meets_gate(queries, kc), want_ok)
+    mixed = MultiSearch(db, queries, kc, 40, proposer=prop)
+    mixed.run()
+    plain = MultiSearch(db, queries, kc, 40)
     plain.run()
-    ms.run()  # falls back to the plain prefilter
-    assert torch.equal(ms.cand, plain.cand) and torch.equal(ms.hit_sc, plain.hit_sc)
+    learned_only = MultiSearch(db, queries, kc, 40, proposer=Proposer(tab, group=16))
+    learned_only.run()
+    for i in range(3):
+        c = slice(int(mixed.c_off_h[i]), int(mixed.c_off_h[i + 1]))
+        ref = learned_only if want_ok[i] else plain
+        assert torch.equal(mixed.cand[c], ref.cand[c]), i
+        assert torch.equal(mixed.cs[c], ref.cs[c]), i
+    assert cal1["k_by_len"] == cal["k_by_len"]
