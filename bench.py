@@ -677,14 +677,14 @@ def main() -> int:
             del allsc, good, gh
             log(json.dumps(r))
             rec.append(r)
+        def mean_of(key):
+            v = [r[key] for r in rec if key in r]
+            return round(float(np.mean(v)), 1) if v else None
+
         out = {"table": build, "sweep": res, "recall_sample": rec,
                "learned_ms_per_query": res["ms_per_query"],
-               "plain_ms_per_query_at_0.99": round(float(np.mean([r["plain_ms_at_0.99"]
-                                                                  for r in rec
-                                                                  if "plain_ms_at_0.99" in r])), 1),
-               "learned_ms_per_query_at_calibrated": round(float(np.mean(
-                   [r["learned_ms_at_calibrated"] for r in rec
-                    if "learned_ms_at_calibrated" in r])), 1),
+               "plain_ms_per_query_at_0.99": mean_of("plain_ms_at_0.99"),
+               "learned_ms_per_query_at_calibrated": mean_of("learned_ms_at_calibrated"),
                "calibration": {k_: v for k_, v in cal.items() if k_ != "per_query"},
                "learned_recall_at_calibrated_min": min(r["learned_recall_at_calibrated"]
                                                        for r in rec),
@@ -707,8 +707,9 @@ def main() -> int:
                                                   for r in rec]))}
         if plain_batched:
             out["plain_batched_ms_per_query_top_prefilter"] = plain_batched["ms_per_query"]
-        out["speedup_vs_plain_at_0.99"] = round(out["plain_ms_per_query_at_0.99"] /
-                                                out["learned_ms_per_query_at_calibrated"], 1)
+        if out["plain_ms_per_query_at_0.99"] and out["learned_ms_per_query_at_calibrated"]:
+            out["speedup_vs_plain_at_0.99"] = round(out["plain_ms_per_query_at_0.99"] /
+                                                    out["learned_ms_per_query_at_calibrated"], 1)
         log(json.dumps({k_: v for k_, v in out.items() if k_ not in ("sweep", "recall_sample")}))
         del ms
         torch.cuda.empty_cache()
