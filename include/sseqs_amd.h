@@ -325,6 +325,11 @@ int sseqs_enc_to_fp8(const float* e, int64_t n, float scale, uint8_t* out, hipSt
  * most 128), on MFMA fp8. */
 int sseqs_propose(const uint8_t* table, int64_t n, const uint8_t* q8, int G, void* scores,
                   int64_t ld, hipStream_t stream);
+/* out[j] = max over r < nrows of S[r * ld + j] for j < n (fp16; ld >= n and a
+ * multiple of 8, S and out 16-byte aligned): a query's score over its
+ * window rows. */
+int sseqs_rows_max(const void* S, int64_t ld, int nrows, int64_t n, void* out,
+                   hipStream_t stream);
 
 /* ------------------------------------------------------- a3m (host) -- */
 

@@ -626,6 +626,34 @@ __global__ __launch_bounds__(256) void k_propose(const uint8_t* __restrict__ T, 
     }
 }
 
+// out[j] = max over r < nrows of S[r * ld + j], j < n (fp16; ld a multiple
+// of 8 and S, out 16-byte aligned: 8 scores per thread and step).
+__global__ __launch_bounds__(256) void k_rows_max(const _Float16* __restrict__ S, int64_t ld,
+                                                  int nrows, int64_t n,
+                                                  _Float16* __restrict__ out) {
+    const int64_t n8 = n / 8;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c * 8 < n;
+         c += (int64_t)gridDim.x * blockDim.x) {
+        if (c < n8) {
+            uint4 v = *reinterpret_cast<const uint4*>(S + c * 8);
+            _Float16* m = reinterpret_cast<_Float16*>(&v);
+            for (int r = 1; r < nrows; ++r) {
+                uint4 w = *reinterpret_cast<const uint4*>(S + r * ld + c * 8);
+                const _Float16* x = reinterpret_cast<const _Float16*>(&w);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) m[e] = x[e] > m[e] ? x[e] : m[e];
+            }
+            *reinterpret_cast<uint4*>(out + c * 8) = v;
+        } else {
+            for (int64_t j = c * 8; j < n; ++j) {
+                _Float16 m = S[j];
+                for (int r = 1; r < nrows; ++r) m = S[r * ld + j] > m ? S[r * ld + j] : m;
+                out[j] = m;
+            }
+        }
+    }
+}
+
 }  // namespace sseqs
 
 using namespace sseqs;
@@ -780,6 +808,18 @@ extern "C" int sseqs_enc_to_fp8(const float* e, int64_t n, float scale, uint8_t*
     hipLaunchKernelGGL(k_to_fp8, dim3((unsigned)ceil_div(n / 4, 256)), dim3(256), 0, stream, e, n,
                        scale, out);
     return check_launch("enc_to_fp8");
+}
+
+extern "C" int sseqs_rows_max(const void* S, int64_t ld, int nrows, int64_t n, void* out,
+                              hipStream_t stream) {
+    if (n <= 0) return 0;
+    if (nrows < 1) return fail(-1, "rows_max: need at least one row");
+    if (ld < n || ld % 8) return fail(-1, "rows_max: ld must be >= n and a multiple of 8");
+    if ((reinterpret_cast<uintptr_t>(S) | reinterpret_cast<uintptr_t>(out)) & 15)
+        return fail(-1, "rows_max: S and out must be 16-byte aligned");
+    hipLaunchKernelGGL(k_rows_max, dim3(grid_cap(ceil_div(n, 8), 256)), dim3(256), 0, stream,
+                       static_cast<const _Float16*>(S), ld, nrows, n, static_cast<_Float16*>(out));
+    return check_launch("rows_max");
 }
 
 extern "C" int sseqs_propose(const uint8_t* table, int64_t n, const uint8_t* q8, int G,

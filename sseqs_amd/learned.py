@@ -592,38 +592,110 @@ def _ld(n: int) -> int:
 
 class Proposer:
     """Candidate selection for multi.MultiSearch from the embedding table:
-    per query, the top kc sequences by e_q . e_t (FP8 MFMA), in ascending
-    index order.  After ``calibrate``, a query whose candidate count does
-    not reach the recall gate (``meets_gate``) is selected by the plain
-    prefilter instead."""
+    per query, the top kc sequences by its proposal score, in ascending
+    index order.  A query's score of a sequence is the largest e_r . e_t
+    (FP8 MFMA) over its embedding rows: the whole query and, past ``window``
+    residues, windows of ``window`` residues every ``stride`` (a mean-pooled
+    embedding of a long query is diluted against a homolog that covers only
+    part of it; a window is not).  After ``calibrate``, a query whose
+    candidate count does not reach the recall gate (``meets_gate``) is
+    selected by the plain prefilter instead."""
 
-    def __init__(self, table: EmbeddingTable, group: int = 64, gate: float = 0.99):
+    def __init__(self, table: EmbeddingTable, group: int = 128, gate: float = 0.99,
+                 window: int = 384, stride: int = 192):
+        if group % 16 or not 16 <= group <= 128:
+            raise ValueError("group: a multiple of 16 in 16..128 rows")
         self.table = table
         self.group = group
         self.gate = gate
+        self.window, self.stride = int(window), int(stride)
         self.use_learned = True
         self.calibration: dict | None = None
         self.k_by_len: list[tuple[int, int, int]] | None = None
         self.enforce_gate = True
-        self._scores = None
+        self._scores = None  # [group, ld] scores of one launch's rows
+        self._best = None    # [1, ld] a query's maximum over its rows
 
-    def scores_for(self, queries: list[str]) -> torch.Tensor:
-        """Proposal scores (fp16 [len(queries), n]) of up to 128 queries."""
+    def rows_of(self, queries: list[str]) -> tuple[list[str], np.ndarray]:
+        """(embedding row texts, rows per query): each query whole, then,
+        when longer than ``window``, its windows of ``window`` residues
+        starting every ``stride`` (the last one ending at the query's end)."""
+        rows, count = [], np.zeros(len(queries), np.int64)
+        for i, q in enumerate(queries):
+            rows.append(q)
+            count[i] = 1
+            if self.window > 0 and len(q) > self.window:
+                starts = list(range(0, len(q) - self.window + 1, self.stride))
+                if starts[-1] != len(q) - self.window:
+                    starts.append(len(q) - self.window)
+                rows += [q[a: a + self.window] for a in starts]
+                count[i] += len(starts)
+        return rows, count
+
+    def _groups(self, queries: list[str]) -> list[tuple]:
+        """Launch groups: (q0, q1, q8 rows [Gp, 256], first row, row count)
+        for consecutive queries q0..q1-1 whose rows fit ``group``."""
         tab = self.table
         dev = tab.table.device
         st = stream_of(dev)
-        e = tab.enc.embed_strings(queries)
-        Gp = (len(queries) + 15) // 16 * 16
-        if Gp > 128:
-            raise ValueError("at most 128 queries per call")
-        q = torch.zeros((Gp, DIM), dtype=torch.float32, device=dev)
-        q[: len(queries)] = e
-        q8 = torch.empty((Gp, DIM), dtype=torch.uint8, device=dev)
-        call("sseqs_enc_to_fp8", ptr(q), q.numel(), FP8_SCALE, ptr(q8), st)
-        ld = _ld(tab.n)
-        out = torch.empty((Gp, ld), dtype=torch.float16, device=dev)
-        call("sseqs_propose", ptr(tab.table), tab.n, ptr(q8), Gp, ptr(out), ld, st)
-        return out[: len(queries), : tab.n]
+        rows, count = self.rows_of(queries)
+        if int(count.max(initial=1)) > self.group:
+            raise ValueError("a query has more window rows than a group holds")
+        e = tab.enc.embed_strings(rows) if rows else None
+        first = np.concatenate([[0], np.cumsum(count)[:-1]]).astype(np.int64)
+        groups, q0 = [], 0
+        while q0 < len(queries):
+            q1 = q0
+            while q1 < len(queries) and first[q1] + count[q1] - first[q0] <= self.group:
+                q1 += 1
+            r0, r1 = int(first[q0]), int(first[q1 - 1] + count[q1 - 1])
+            Gp = (r1 - r0 + 15) // 16 * 16
+            q = torch.zeros((Gp, DIM), dtype=torch.float32, device=dev)
+            q[: r1 - r0] = e[r0:r1]
+            q8 = torch.empty((Gp, DIM), dtype=torch.uint8, device=dev)
+            call("sseqs_enc_to_fp8", ptr(q), q.numel(), FP8_SCALE, ptr(q8), st)
+            groups.append((q0, q1, q8, first[q0:q1] - r0, count[q0:q1].copy()))
+            q0 = q1
+        return groups
+
+    def _buffers(self, dev) -> int:
+        """Allocate the score buffers (group rows of n scores, 51 GB for 128
+        rows of 200 M) once; returns their row stride."""
+        ld = _ld(self.table.n)
+        if self._scores is None or self._scores.shape[1] != ld:
+            self._scores = torch.empty((self.group, ld), dtype=torch.float16, device=dev)
+            self._best = torch.empty((1, ld), dtype=torch.float16, device=dev)
+        return ld
+
+    def _propose(self, q8: torch.Tensor, st) -> None:
+        tab = self.table
+        call("sseqs_propose", ptr(tab.table), tab.n, ptr(q8), q8.shape[0], ptr(self._scores),
+             self._scores.shape[1], st)
+
+    def _query_row(self, first: int, count: int, out: int | None, st) -> int:
+        """Address of a query's score row: its only row, or the maximum over
+        its rows written to ``out`` (default: the scratch row)."""
+        ld = self._scores.shape[1]
+        src = self._scores.data_ptr() + 2 * first * ld
+        if count == 1 and out is None:
+            return src
+        dst = out if out is not None else self._best.data_ptr()
+        call("sseqs_rows_max", src, ld, int(count), self.table.n, dst, st)
+        return dst
+
+    def scores_for(self, queries: list[str]) -> torch.Tensor:
+        """Proposal scores (fp16 [len(queries), n], a view of a padded
+        matrix) of the queries: each one's maximum over its rows."""
+        tab = self.table
+        dev = tab.table.device
+        st = stream_of(dev)
+        ld = self._buffers(dev)
+        out = torch.empty((len(queries), ld), dtype=torch.float16, device=dev)
+        for q0, q1, q8, first, count in self._groups(queries):
+            self._propose(q8, st)
+            for j in range(q1 - q0):
+                self._query_row(int(first[j]), int(count[j]), out[q0 + j].data_ptr(), st)
+        return out[:, : tab.n]
 
     # query-length classes of the calibrated operating point
     LEN_EDGES = (0, 100, 200, 350, 500, 700, 2049)
@@ -680,26 +752,11 @@ class Proposer:
         return out
 
     def prepare(self, ms) -> None:
-        """Set-up of a sweep (before any capture): the queries' fp8
-        embedding rows, in groups of at most ``group`` (each padded to a
-        multiple of 16 with zero rows)."""
-        tab = self.table
-        st = stream_of(ms.dev)
+        """Set-up of a sweep (before any capture): the fp8 embedding rows of
+        its queries in launch groups, and the score buffers."""
         t0 = time.perf_counter()
-        e = tab.enc.embed_strings(ms.texts)
-        groups = []
-        for g0 in range(0, ms.nq, self.group):
-            g1 = min(ms.nq, g0 + self.group)
-            Gp = (g1 - g0 + 15) // 16 * 16
-            q = torch.zeros((Gp, DIM), dtype=torch.float32, device=ms.dev)
-            q[: g1 - g0] = e[g0:g1]
-            q8 = torch.empty((Gp, DIM), dtype=torch.uint8, device=ms.dev)
-            call("sseqs_enc_to_fp8", ptr(q), q.numel(), FP8_SCALE, ptr(q8), st)
-            groups.append((g0, g1, q8))
-        ms.learned_groups = groups
-        if self._scores is None or self._scores.shape[1] != _ld(tab.n):
-            self._scores = torch.empty((self.group, _ld(tab.n)), dtype=torch.float16,
-                                       device=ms.dev)
+        self._buffers(ms.dev)
+        ms.learned_groups = self._groups(ms.texts)
         torch.cuda.synchronize(ms.dev)
         ms.query_embed_s = time.perf_counter() - t0
 
@@ -731,18 +788,17 @@ class Proposer:
         st = stream_of(ms.dev)
         if getattr(ms, "learned_groups", None) is None:
             self.prepare(ms)
-        for g0, g1, q8 in ms.learned_groups:
-            if not ok[g0:g1].any():
+        for q0, q1, q8, first, count in ms.learned_groups:
+            if not ok[q0:q1].any():
                 continue
-            ld = self._scores.shape[1]
-            call("sseqs_propose", ptr(tab.table), tab.n, ptr(q8), q8.shape[0], ptr(self._scores),
-                 ld, st)
-            for j in range(g1 - g0):
-                i = g0 + j
+            self._propose(q8, st)
+            for j in range(q1 - q0):
+                i = q0 + j
                 if ok[i]:
-                    call("sseqs_topk", self._scores.data_ptr() + 2 * j * ld, _lib.F16, tab.n,
-                         int(ms.kc_q[i]), 0, ms.cand.data_ptr() + 8 * int(ms.c_off_h[i]),
-                         ptr(ms.sel_ws), ms.sel_ws.numel(), st)
+                    src = self._query_row(int(first[j]), int(count[j]), None, st)
+                    call("sseqs_topk", src, _lib.F16, tab.n, int(ms.kc_q[i]), 0,
+                         ms.cand.data_ptr() + 8 * int(ms.c_off_h[i]), ptr(ms.sel_ws),
+                         ms.sel_ws.numel(), st)
 
 
 def recall_curve(scores: torch.Tensor, good: torch.Tensor, k: int, grid=(),

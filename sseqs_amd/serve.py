@@ -100,6 +100,8 @@ class SearchEngine:
         self.proposer = proposer
         self.hit_width = int(hit_width)
         self.max_wait_s = float(max_wait_s)
+        if proposer is not None and hasattr(proposer, "_buffers"):
+            proposer._buffers(db.device)  # its score buffers before the budget is taken
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info(db.device)
             hbm_budget = int(0.5 * free) // self.in_flight

@@ -15,8 +15,9 @@ definition.
 * the FP8 table holds e4m3 of 16 * e, and rows built in chunks equal the
   same sequences embedded in one batch, bit for bit;
 * proposal scores equal the fp32 dot products of the fp8 operands within
-  fp16 rounding, and the proposed candidates are the exact top-k of those
-  scores; the learned cascade's SW scores are the oracle's;
+  fp16 rounding, a query's score is the exact maximum over its window rows,
+  and the proposed candidates are the exact top-k of those scores; the
+  learned cascade's SW scores are the oracle's;
 * calibration recall equals the share of the oracle's exhaustive top k
   among the top K proposals, and a failed gate hands selection back to the
   plain prefilter.
@@ -206,15 +207,22 @@ def test_table_and_proposal(orc, enc, dev):
     dec = tab.table[:500].cpu().view(torch.float8_e4m3fn).float()
     want = (FP8_SCALE * enc.embed_strings(db.strings(torch.arange(500, device=dev), 2048))).cpu()
     assert torch.allclose(dec, want.to(torch.float8_e4m3fn).float(), atol=0, rtol=0)
-    # proposal scores and candidates
-    ms = MultiSearch(db, queries, top_prefilter=600, top_hits=50, proposer=Proposer(tab, group=16))
+    # proposal scores and candidates: every embedding row of the queries (the
+    # 510-residue query adds two 384-residue windows), and a query's score
+    # is the maximum over its rows
+    prop16 = Proposer(tab, group=16)
+    rows, count = prop16.rows_of(queries)
+    assert list(count) == [1, 1, 3] and rows[3] == queries[2][:384] and rows[4] == queries[2][-384:]
+    ms = MultiSearch(db, queries, top_prefilter=600, top_hits=50, proposer=prop16)
     ms.run()
-    sc = ms.proposer._scores[:3, : db.n].float().cpu()
-    qe = ms.proposer.table.enc.embed_strings(queries)
+    sc_rows = prop16._scores[:5, : db.n].float().cpu()
+    qe = enc.embed_strings(rows)
     qq = (FP8_SCALE * qe).cpu().to(torch.float8_e4m3fn).float()
     tt = tab.table.cpu().view(torch.float8_e4m3fn).float()
     ref = qq @ tt.T
-    assert torch.allclose(sc, ref.half().float(), rtol=2e-3, atol=2e-3)
+    assert torch.allclose(sc_rows, ref.half().float(), rtol=2e-3, atol=2e-3)
+    sc = torch.stack([sc_rows[0], sc_rows[1], sc_rows[2:5].max(dim=0).values])
+    assert torch.equal(prop16.scores_for(queries).float().cpu(), sc)
     for i in range(3):
         cand = ms.cand[i * 600:(i + 1) * 600].cpu().numpy()
         top = orc.topk_desc(sc[i].numpy().astype(np.float64), 600)
@@ -232,12 +240,21 @@ def test_table_and_proposal(orc, enc, dev):
     tab2 = EmbeddingTable(enc, db2)
     s_pad = Proposer(tab2).scores_for(queries).float().cpu()
     tt2 = tab2.table.cpu().view(torch.float8_e4m3fn).float()
-    assert torch.allclose(s_pad, (qq @ tt2.T).half().float(), rtol=2e-3, atol=2e-3)
+    r2 = (qq @ tt2.T).half().float()
+    want2 = torch.stack([r2[0], r2[1], r2[2:5].max(dim=0).values])
+    assert torch.allclose(s_pad, want2, rtol=2e-3, atol=2e-3)
+    s_whole = Proposer(tab2, window=0).scores_for(queries).float().cpu()
+    qw = enc.embed_strings(queries)
     q8 = torch.zeros((16, 256), dtype=torch.uint8, device=dev)
-    call("sseqs_enc_to_fp8", ptr(qe), qe.numel(), FP8_SCALE, ptr(q8), stream_of(dev))
+    call("sseqs_enc_to_fp8", ptr(qw), qw.numel(), FP8_SCALE, ptr(q8), stream_of(dev))
     raw = torch.full((16, 1003), float("nan"), dtype=torch.float16, device=dev)
     call("sseqs_propose", ptr(tab2.table), 1003, ptr(q8), 16, ptr(raw), 1003, stream_of(dev))
-    assert torch.equal(raw[:3].float().cpu(), s_pad)
+    assert torch.equal(raw[:3].float().cpu(), s_whole)
+    # the row maximum against torch's, on a stride padded to 8 with a tail
+    g = torch.randn((5, 1008), device=dev).half()
+    mx = torch.empty(1008, dtype=torch.float16, device=dev)
+    call("sseqs_rows_max", ptr(g), 1008, 5, 1003, ptr(mx), stream_of(dev))
+    assert torch.equal(mx[:1003], g[:, :1003].max(dim=0).values)
     # calibration against exhaustive SW (oracle scores of every sequence):
     # recall at K is the share of the exhaustive top k among the top K
     # proposals, and the gate decides between the learned and plain selection
