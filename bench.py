@@ -208,13 +208,13 @@ def main() -> int:
     ap.add_argument("--no-learned", action="store_true", help="skip the learned prefilter")
     ap.add_argument("--learned-only", action="store_true",
                     help="run only the learned prefilter measurement")
-    ap.add_argument("--recall-queries", type=int, default=16,
+    ap.add_argument("--recall-queries", type=int, default=12,
                     help="batched queries whose Recall@4096 is measured against exhaustive SW")
     ap.add_argument("--recall-timed", type=int, default=2,
                     help="of those, how many get the plain cascade timed at its 0.99 point")
-    ap.add_argument("--serve-requests", type=int, default=400,
+    ap.add_argument("--serve-requests", type=int, default=300,
                     help="requests of the serving measurement at the high rate (0 skips it)")
-    ap.add_argument("--serve-plain-requests", type=int, default=8)
+    ap.add_argument("--serve-plain-requests", type=int, default=6)
     ap.add_argument("--serve-rate-high", type=float, default=200.0)
     ap.add_argument("--serve-rate-low", type=float, default=20.0)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
