@@ -3,7 +3,8 @@
 // Used wherever the reference calls torch.topk / torch.sort on the search
 // path (sseqs/msa.py:297, 307, 324, 335).  Scores become order-preserving
 // 32-bit keys; three histogram passes (11 + 11 + 10 bits, most significant
-// first) find the k-th largest key T and how many keys equal to T are taken;
+// first; two, 11 + 8, for fp16 scores, whose keys end in 13 cleared bits) find
+// the k-th largest key T and how many keys equal to T are taken;
 // a compaction pass then writes every index with key > T plus the first
 // ones with key == T, in ascending index order.  The optional sort orders
 // them by (key desc, index asc) with a bitonic sort in LDS.  The tie rule
@@ -32,7 +33,10 @@ __device__ inline uint32_t key_f32(float x) {
 template <int DT>
 __device__ inline uint32_t load_key(const void* s, int64_t i) {
     if constexpr (DT == SSEQS_F32) return key_f32(static_cast<const float*>(s)[i]);
-    if constexpr (DT == SSEQS_F16) return key_f32((float)static_cast<const _Float16*>(s)[i]);
+    // an fp16 value as fp32 has 13 constant low bits (zeros, ones once
+    // complemented for a negative value): clearing them keeps the order
+    if constexpr (DT == SSEQS_F16)
+        return key_f32((float)static_cast<const _Float16*>(s)[i]) & ~0x1FFFu;
     return (uint32_t) static_cast<const int32_t*>(s)[i] ^ 0x80000000u;
 }
 
@@ -509,10 +513,15 @@ extern "C" int sseqs_topk(const void* scores, int dtype, int64_t n, int64_t k, i
 
     hipLaunchKernelGGL(k_sel_init, dim3(1), dim3(256), 0, stream, hist, st, (uint32_t)k);
     if (int r = check_launch("topk init")) return r;
-    const int shifts[3] = {21, 10, 0}, bits[3] = {11, 11, 10};
+    // keys of fp16 scores have their low 13 bits zero: two passes fix the rest
+    const int shifts32[3] = {21, 10, 0}, bits32[3] = {11, 11, 10};
+    const int shifts16[2] = {21, 13}, bits16[2] = {11, 8};
+    const bool half = dtype == SSEQS_F16;
+    const int* shifts = half ? shifts16 : shifts32;
+    const int* bits = half ? bits16 : bits32;
     int64_t hb = ceil_div(n, 256 * 16);
     unsigned hgrid = (unsigned)(hb < 4096 ? (hb > 0 ? hb : 1) : 4096);
-    for (int p = 0; p < 3; ++p) {
+    for (int p = 0; p < (half ? 2 : 3); ++p) {
         SEL_DISPATCH(k_sel_hist, dim3(hgrid), dim3(256), 0, stream, scores, n, st, hist,
                            shifts[p], bits[p]);
         if (int r = check_launch("topk hist")) return r;
