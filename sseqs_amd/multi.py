@@ -43,6 +43,7 @@ from .to_a3m import a3m_text, a3m_text_rows
 TUNING_PATH = Path(__file__).resolve().parent / "tuning.json"
 
 # stripe-width classes that share SW queue parameters
+SW_STREAMS = 2  # streams the SW queues of different stripe widths alternate over
 SW_CLASSES = ((1, 8), (9, 16), (17, 32))
 DEFAULT_SW = {"item": 1024, "blocks_per_cu": 2}
 N_CU = 256
@@ -176,6 +177,10 @@ class MultiSearch:
         self.learned_groups = None
         if proposer is not None and hasattr(proposer, "prepare"):
             proposer.prepare(self)  # query-side work with host copies, before any capture
+        # streams of the SW queues (default priority) and their fork/join events
+        self._sw_streams = [torch.cuda.Stream(device=dev) for _ in range(SW_STREAMS)]
+        self._sw_fork = torch.cuda.Event()
+        self._sw_join = [torch.cuda.Event() for _ in self._sw_streams]
         self.graph: torch.cuda.CUDAGraph | None = None
 
     def set_sw_tuning(self, tuning: dict) -> None:
@@ -220,13 +225,28 @@ class MultiSearch:
              self.sel_ws.numel(), st)
 
     def _sw(self) -> None:
-        st = stream_of(self.dev)
+        """The SW queues, one per stripe width, alternating over SW_STREAMS
+        streams forked from and joined back to the current stream, so that
+        one queue's tail overlaps the next queue's start; then the queries
+        past 1024 residues on the current stream."""
+        main = torch.cuda.current_stream(self.dev)
+        st = main.cuda_stream
         db = self.db
-        for b in self.buckets:
+        side = self._sw_streams if len(self.buckets) > 1 else []
+        if side:
+            self._sw_fork.record(main)
+            for s_ in side:
+                s_.wait_event(self._sw_fork)
+        for idx, b in enumerate(self.buckets):
             call("sseqs_db_sw_batch", ptr(db.packed), ptr(db.seq_off), ptr(db.seq_len),
                  ptr(self.qcodes), ptr(self.q_off), ptr(self.cand), ptr(self.c_off),
                  ptr(b["item_q"]), ptr(b["item_j"]), b["n"], b["item"], b["tpt"], b["grid"],
-                 self.go, self.ge, ptr(self.cs), ptr(b["counter"]), st)
+                 self.go, self.ge, ptr(self.cs), ptr(b["counter"]),
+                 side[idx % len(side)].cuda_stream if side else st)
+        if side:
+            for s_, e in zip(side, self._sw_join):
+                e.record(s_)
+                main.wait_event(e)
         for i in self.long_queries:  # past 1024 residues: the 32-bit kernel
             q = self.qs[i]
             c0 = int(self.c_off_h[i])
