@@ -606,11 +606,17 @@ __global__ __launch_bounds__(256) void k_propose(const uint8_t* __restrict__ T, 
                     acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(fa[i][s], fb, acc[i], 0,
                                                                         0, 0);
             }
+            // rows 4g .. 4g+3 of each row tile are consecutive: one 8-byte LDS store
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < 2; ++i) {
+                union {
+                    _Float16 h[4];
+                    uint2 u;
+                } v;
 #pragma unroll
-                for (int x = 0; x < 4; ++x)
-                    sS[r * PROP_ROWS + 32 * w + 16 * i + 4 * g + x] = (_Float16)acc[i][x];
+                for (int x = 0; x < 4; ++x) v.h[x] = (_Float16)acc[i][x];
+                *reinterpret_cast<uint2*>(&sS[r * PROP_ROWS + 32 * w + 16 * i + 4 * g]) = v.u;
+            }
             __syncthreads();
             const int64_t row = b0 + 8 * oc;
             _Float16* dst = S + (int64_t)(q0 + oq) * ldS + row;

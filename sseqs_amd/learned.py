@@ -601,7 +601,7 @@ class Proposer:
     candidate count does not reach the recall gate (``meets_gate``) is
     selected by the plain prefilter instead."""
 
-    def __init__(self, table: EmbeddingTable, group: int = 128, gate: float = 0.99,
+    def __init__(self, table: EmbeddingTable, group: int = 64, gate: float = 0.99,
                  window: int = 384, stride: int = 192):
         if group % 16 or not 16 <= group <= 128:
             raise ValueError("group: a multiple of 16 in 16..128 rows")
@@ -633,8 +633,10 @@ class Proposer:
         return rows, count
 
     def _groups(self, queries: list[str]) -> list[tuple]:
-        """Launch groups: (q0, q1, q8 rows [Gp, 256], first row, row count)
-        for consecutive queries q0..q1-1 whose rows fit ``group``."""
+        """Launch groups: (query indices, q8 rows [Gp, 256], first row of
+        each query in the group, its row count).  Queries are packed first
+        fit by decreasing row count, so that few launches (each a read of the
+        whole table) hold all rows; a query's rows stay in one group."""
         tab = self.table
         dev = tab.table.device
         st = stream_of(dev)
@@ -643,23 +645,33 @@ class Proposer:
             raise ValueError("a query has more window rows than a group holds")
         e = tab.enc.embed_strings(rows) if rows else None
         first = np.concatenate([[0], np.cumsum(count)[:-1]]).astype(np.int64)
-        groups, q0 = [], 0
-        while q0 < len(queries):
-            q1 = q0
-            while q1 < len(queries) and first[q1] + count[q1] - first[q0] <= self.group:
-                q1 += 1
-            r0, r1 = int(first[q0]), int(first[q1 - 1] + count[q1 - 1])
-            Gp = (r1 - r0 + 15) // 16 * 16
+        bins: list[list[int]] = []
+        fill: list[int] = []
+        for i in sorted(range(len(queries)), key=lambda i: (-int(count[i]), i)):
+            for g in range(len(bins)):
+                if fill[g] + count[i] <= self.group:
+                    bins[g].append(i)
+                    fill[g] += int(count[i])
+                    break
+            else:
+                bins.append([i])
+                fill.append(int(count[i]))
+        groups = []
+        for members in bins:
+            qidx = np.array(sorted(members), dtype=np.int64)
+            cnt = count[qidx].copy()
+            loc = np.concatenate([[0], np.cumsum(cnt)[:-1]]).astype(np.int64)
+            src = np.concatenate([np.arange(first[i], first[i] + count[i]) for i in qidx])
+            Gp = (int(cnt.sum()) + 15) // 16 * 16
             q = torch.zeros((Gp, DIM), dtype=torch.float32, device=dev)
-            q[: r1 - r0] = e[r0:r1]
+            q[: src.size] = e[torch.from_numpy(src).to(dev)]
             q8 = torch.empty((Gp, DIM), dtype=torch.uint8, device=dev)
             call("sseqs_enc_to_fp8", ptr(q), q.numel(), FP8_SCALE, ptr(q8), st)
-            groups.append((q0, q1, q8, first[q0:q1] - r0, count[q0:q1].copy()))
-            q0 = q1
+            groups.append((qidx, q8, loc, cnt))
         return groups
 
     def _buffers(self, dev) -> int:
-        """Allocate the score buffers (group rows of n scores, 51 GB for 128
+        """Allocate the score buffers (group rows of n scores, 25.6 GB for 64
         rows of 200 M) once; returns their row stride."""
         ld = _ld(self.table.n)
         if self._scores is None or self._scores.shape[1] != ld:
@@ -691,10 +703,10 @@ class Proposer:
         st = stream_of(dev)
         ld = self._buffers(dev)
         out = torch.empty((len(queries), ld), dtype=torch.float16, device=dev)
-        for q0, q1, q8, first, count in self._groups(queries):
+        for qidx, q8, first, count in self._groups(queries):
             self._propose(q8, st)
-            for j in range(q1 - q0):
-                self._query_row(int(first[j]), int(count[j]), out[q0 + j].data_ptr(), st)
+            for j, i in enumerate(qidx):
+                self._query_row(int(first[j]), int(count[j]), out[int(i)].data_ptr(), st)
         return out[:, : tab.n]
 
     # query-length classes of the calibrated operating point
@@ -788,12 +800,12 @@ class Proposer:
         st = stream_of(ms.dev)
         if getattr(ms, "learned_groups", None) is None:
             self.prepare(ms)
-        for q0, q1, q8, first, count in ms.learned_groups:
-            if not ok[q0:q1].any():
+        for qidx, q8, first, count in ms.learned_groups:
+            if not ok[qidx].any():
                 continue
             self._propose(q8, st)
-            for j in range(q1 - q0):
-                i = q0 + j
+            for j, i in enumerate(qidx):
+                i = int(i)
                 if ok[i]:
                     src = self._query_row(int(first[j]), int(count[j]), None, st)
                     call("sseqs_topk", src, _lib.F16, tab.n, int(ms.kc_q[i]), 0,
