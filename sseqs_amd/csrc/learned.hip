@@ -560,14 +560,17 @@ __global__ void k_to_fp8(const float* __restrict__ e, int64_t n, float scale,
 //
 // S[q][s] = sum_c T[s][c] * Q[q][c] over the fp8 table T (n x 256) and G
 // query rows Q (G x 256, G a multiple of 16, at most 128), written as fp16
-// to S[q * ldS + s].  A block takes 128 consecutive table rows per step
-// (4 waves x 32 rows) against all G queries held in LDS.  MFMA 16x16x32
-// fp8: lane (r = lane % 16, g = lane / 16) supplies row r, bytes 8g .. 8g+7
-// of the 32-byte K slice; result element i of lane (c, g) is row 4g + i,
-// column c.  Each group of 16 queries' 16 x 128 scores goes through LDS so
-// that the block writes every query's 256 contiguous bytes with 16-byte
-// stores (the score matrix is a quarter of the traffic).
-constexpr int PROP_ROWS = 128;
+// to S[q * ldS + s].  A block takes PROP_ROWS consecutive table rows per
+// step (4 waves x 32 PROP_SUB rows, kept in registers for every query group)
+// against all G queries held in LDS.  MFMA 16x16x32 fp8: lane (r = lane %
+// 16, g = lane / 16) supplies row r, bytes 8g .. 8g+7 of the 32-byte K slice;
+// result element i of lane (c, g) is row 4g + i, column c.  Each group of 16
+// queries' 16 x PROP_ROWS scores goes through LDS so that the block writes
+// every query's 2 PROP_ROWS contiguous bytes with 16-byte stores: the writes
+// (a third of the traffic at 64 query rows) run at a fraction of the reads'
+// rate in short runs.
+constexpr int PROP_SUB = 2;                // 32-row tiles per wave and step
+constexpr int PROP_ROWS = 128 * PROP_SUB;  // table rows per block and step
 
 __global__ __launch_bounds__(256) void k_propose(const uint8_t* __restrict__ T, int64_t n,
                                                  const uint8_t* __restrict__ Q, int G,
@@ -579,53 +582,67 @@ __global__ __launch_bounds__(256) void k_propose(const uint8_t* __restrict__ T, 
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r = lane & 15, g = lane >> 4;
-    // the cooperative store: thread -> (query oq of the group, rows 8 oc .. 8 oc + 7)
+    // the cooperative store: thread -> (query oq of the group, rows 8 oc .. 8 oc + 7
+    // of each 128-row part)
     const int oq = threadIdx.x >> 4, oc = threadIdx.x & 15;
     const bool vec = (ldS & 7) == 0 && (reinterpret_cast<uintptr_t>(S) & 15) == 0;
     for (int64_t b0 = (int64_t)blockIdx.x * PROP_ROWS; b0 < n;
          b0 += (int64_t)gridDim.x * PROP_ROWS) {
-        const int64_t t0 = b0 + 32 * w;
-        // 32 table rows (two MFMA row tiles) of this wave, all 8 K-slices
-        long fa[2][8];
+        const int64_t t0 = b0 + 32 * PROP_SUB * w;
+        // this wave's 32 PROP_SUB table rows (two MFMA row tiles per 32), all
+        // 8 K-slices, kept for every query group
+        long fa[PROP_SUB][2][8];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            const int64_t row = min(t0 + 16 * i + r, n - 1);
+        for (int u = 0; u < PROP_SUB; ++u)
 #pragma unroll
-            for (int s = 0; s < 8; ++s)
-                fa[i][s] = *reinterpret_cast<const long*>(T + row * 256 + 32 * s + 8 * g);
-        }
+            for (int i = 0; i < 2; ++i) {
+                const int64_t row = min(t0 + 32 * u + 16 * i + r, n - 1);
+#pragma unroll
+                for (int s = 0; s < 8; ++s)
+                    fa[u][i][s] = *reinterpret_cast<const long*>(T + row * 256 + 32 * s + 8 * g);
+            }
         for (int q0 = 0; q0 < G; q0 += 16) {
-            f32x4 acc[2];
+            f32x4 acc[PROP_SUB][2];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) acc[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int u = 0; u < PROP_SUB; ++u)
+#pragma unroll
+                for (int i = 0; i < 2; ++i) acc[u][i] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int s = 0; s < 8; ++s) {
                 const long fb = *reinterpret_cast<const long*>(&sQ[(q0 + r) * 256 + 32 * s + 8 * g]);
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
-                    acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(fa[i][s], fb, acc[i], 0,
-                                                                        0, 0);
+                for (int u = 0; u < PROP_SUB; ++u)
+#pragma unroll
+                    for (int i = 0; i < 2; ++i)
+                        acc[u][i] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8(fa[u][i][s], fb,
+                                                                               acc[u][i], 0, 0, 0);
             }
             // rows 4g .. 4g+3 of each row tile are consecutive: one 8-byte LDS store
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                union {
-                    _Float16 h[4];
-                    uint2 u;
-                } v;
+            for (int u = 0; u < PROP_SUB; ++u)
 #pragma unroll
-                for (int x = 0; x < 4; ++x) v.h[x] = (_Float16)acc[i][x];
-                *reinterpret_cast<uint2*>(&sS[r * PROP_ROWS + 32 * w + 16 * i + 4 * g]) = v.u;
-            }
+                for (int i = 0; i < 2; ++i) {
+                    union {
+                        _Float16 h[4];
+                        uint2 u2;
+                    } v;
+#pragma unroll
+                    for (int x = 0; x < 4; ++x) v.h[x] = (_Float16)acc[u][i][x];
+                    *reinterpret_cast<uint2*>(
+                        &sS[r * PROP_ROWS + 32 * PROP_SUB * w + 32 * u + 16 * i + 4 * g]) = v.u2;
+                }
             __syncthreads();
-            const int64_t row = b0 + 8 * oc;
-            _Float16* dst = S + (int64_t)(q0 + oq) * ldS + row;
-            const _Float16* src = &sS[oq * PROP_ROWS + 8 * oc];
-            if (vec && row + 8 <= n) {
-                *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(src);
-            } else {
-                for (int x = 0; x < 8; ++x)
-                    if (row + x < n) dst[x] = src[x];
+#pragma unroll
+            for (int part = 0; part < PROP_SUB; ++part) {
+                const int64_t row = b0 + 128 * part + 8 * oc;
+                _Float16* dst = S + (int64_t)(q0 + oq) * ldS + row;
+                const _Float16* src = &sS[oq * PROP_ROWS + 128 * part + 8 * oc];
+                if (vec && row + 8 <= n) {
+                    *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(src);
+                } else {
+                    for (int x = 0; x < 8; ++x)
+                        if (row + x < n) dst[x] = src[x];
+                }
             }
             __syncthreads();
         }
@@ -833,7 +850,8 @@ extern "C" int sseqs_propose(const uint8_t* table, int64_t n, const uint8_t* q8,
     if (n <= 0) return 0;
     if (G <= 0 || G % 16 || G > 128) return fail(-1, "propose: G must be 16..128, a multiple of 16");
     if (ld < n) return fail(-1, "propose: ld smaller than n");
-    const unsigned grid = (unsigned)(ceil_div(n, 128) < 8192 ? ceil_div(n, 128) : 8192);
+    const unsigned grid =
+        (unsigned)(ceil_div(n, PROP_ROWS) < 8192 ? ceil_div(n, PROP_ROWS) : 8192);
     hipLaunchKernelGGL(k_propose, dim3(grid), dim3(256), 0, stream, table, n, q8, G,
                        static_cast<_Float16*>(scores), ld);
     return check_launch("propose");
