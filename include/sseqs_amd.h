@@ -330,6 +330,17 @@ int sseqs_propose(const uint8_t* table, int64_t n, const uint8_t* q8, int G, voi
  * window rows. */
 int sseqs_rows_max(const void* S, int64_t ld, int nrows, int64_t n, void* out,
                    hipStream_t stream);
+/* sseqs_propose fused with sseqs_rows_max: output row o (nslot of them, at
+ * most 32) is the maximum of the proposal scores of query rows row_off[o] ..
+ * row_off[o + 1] - 1, taken in ascending row order as sseqs_rows_max takes
+ * them: scores[o * ld + s] (fp16).  G query rows (a multiple of 16, at most
+ * 64); rows outside the runs are padding.  row_off is a host array of
+ * nslot + 1 strictly increasing offsets within 0 .. G; only output rows
+ * 0 .. nslot-1 are written.  A query whose score is the maximum over its
+ * window rows gets one row write per table row instead of one per window. */
+int sseqs_propose_max(const uint8_t* table, int64_t n, const uint8_t* q8, int G,
+                      const int32_t* row_off, int nslot, void* scores, int64_t ld,
+                      hipStream_t stream);
 
 /* ------------------------------------------------------- a3m (host) -- */
 

@@ -76,6 +76,7 @@ SIGNATURES = {
     "sseqs_enc_to_fp8": (_i32, [_vp, _i64, C.c_float, _vp, _vp]),
     "sseqs_propose": (_i32, [_vp, _i64, _vp, _i32, _vp, _i64, _vp]),
     "sseqs_rows_max": (_i32, [_vp, _i64, _i32, _i64, _vp, _vp]),
+    "sseqs_propose_max": (_i32, [_vp, _i64, _vp, _i32, _vp, _i32, _vp, _i64, _vp]),
     "sseqs_topk_ws_bytes": (_sz, [_i64]),
     "sseqs_topk": (_i32, [_vp, _i32, _i64, _i64, _i32, _vp, _vp, _sz, _vp]),
     "sseqs_topk_noalloc_max_n": (_i64, []),

@@ -563,34 +563,68 @@ __global__ void k_to_fp8(const float* __restrict__ e, int64_t n, float scale,
 // to S[q * ldS + s].  A block takes PROP_ROWS consecutive table rows per
 // step (4 waves x 32 PROP_SUB rows, kept in registers for every query group)
 // against all G queries held in LDS.  MFMA 16x16x32 fp8: lane (r = lane %
-// 16, g = lane / 16) supplies row r, bytes 8g .. 8g+7 of the 32-byte K slice;
-// result element i of lane (c, g) is row 4g + i, column c.  Each group of 16
-// queries' 16 x PROP_ROWS scores goes through LDS so that the block writes
-// every query's 2 PROP_ROWS contiguous bytes with 16-byte stores: the writes
-// (a third of the traffic at 64 query rows) run at a fraction of the reads'
-// rate in short runs.
+// 16, g = lane / 16) supplies 8 bytes of row r per 32-byte K slice; result
+// element i of lane (c, g) is row 4g + i, column c.  A dot product does not
+// depend on the order of its terms, so the K slices are taken in the order
+// that lets each lane read 16 bytes at once: slice s of lane (r, g) is bytes
+// 64 (s / 2) + 16 g + 8 (s % 2) of its row, for the table and the query
+// alike (16-byte loads read the table at 6.2 TB/s where 8-byte loads of the
+// natural order reach 4.8).  Each group of 16 queries' 16 x PROP_ROWS scores
+// goes through LDS so that the block writes every query's 2 PROP_ROWS
+// contiguous bytes with 16-byte stores.
 constexpr int PROP_SUB = 2;                // 32-row tiles per wave and step
 constexpr int PROP_ROWS = 128 * PROP_SUB;  // table rows per block and step
+constexpr int PROP_MAX_SLOTS = 32;         // output rows of a MAXQ launch
+static_assert(PROP_ROWS == 256, "the MAXQ fold gives each of 256 threads one table row");
 
+// Orders a wave's LDS accesses across its lanes (no barrier between waves):
+// wavefront-scope release/acquire fences keep the compiler's memory
+// operations on either side, and the hardware runs one wave's LDS
+// instructions in order.
+__device__ __forceinline__ void lds_wave_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Output rows of a MAXQ launch, as runs of query rows: query rows lo .. hi-1
+// are folded, bit q of starts marks the first row of a run, and output row o
+// is the maximum over the rows of run o.
+struct PropSlots {
+    uint64_t starts;
+    int lo, hi;
+};
+
+// MAXQ: the kernel writes, per output row, the maximum of its query rows'
+// scores (taken in ascending row order, as sseqs_rows_max takes them)
+// instead of every query row's scores.  Thread t keeps the running maximum
+// of table row t of the step in a register while it walks the query rows in
+// order, and stores an output row's maximum to LDS once, when its run of
+// query rows ends; no value is read back, so the fold costs one LDS read per
+// query row and score.
+template <bool MAXQ>
 __global__ __launch_bounds__(256) void k_propose(const uint8_t* __restrict__ T, int64_t n,
                                                  const uint8_t* __restrict__ Q, int G,
+                                                 PropSlots slots, int nslot,
                                                  _Float16* __restrict__ S, int64_t ldS) {
-    __shared__ __align__(16) uint8_t sQ[128 * 256];
+    constexpr int QROWS = MAXQ ? 64 : 128;
+    __shared__ __align__(16) uint8_t sQ[QROWS * 256];
     __shared__ __align__(16) _Float16 sS[16 * PROP_ROWS];
+    __shared__ __align__(16) _Float16 sM[MAXQ ? PROP_MAX_SLOTS * PROP_ROWS : 8];
     for (int i = threadIdx.x; i < G * 16; i += 256)
         reinterpret_cast<uint4*>(sQ)[i] = reinterpret_cast<const uint4*>(Q)[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r = lane & 15, g = lane >> 4;
-    // the cooperative store: thread -> (query oq of the group, rows 8 oc .. 8 oc + 7
-    // of each 128-row part)
+    // the cooperative store: thread -> (row oq of 16, scores 8 oc .. 8 oc + 7
+    // of each 128-score part)
     const int oq = threadIdx.x >> 4, oc = threadIdx.x & 15;
     const bool vec = (ldS & 7) == 0 && (reinterpret_cast<uintptr_t>(S) & 15) == 0;
     for (int64_t b0 = (int64_t)blockIdx.x * PROP_ROWS; b0 < n;
          b0 += (int64_t)gridDim.x * PROP_ROWS) {
         const int64_t t0 = b0 + 32 * PROP_SUB * w;
         // this wave's 32 PROP_SUB table rows (two MFMA row tiles per 32), all
-        // 8 K-slices, kept for every query group
+        // 8 K slices, kept for every query group
         long fa[PROP_SUB][2][8];
 #pragma unroll
         for (int u = 0; u < PROP_SUB; ++u)
@@ -598,9 +632,15 @@ __global__ __launch_bounds__(256) void k_propose(const uint8_t* __restrict__ T, 
             for (int i = 0; i < 2; ++i) {
                 const int64_t row = min(t0 + 32 * u + 16 * i + r, n - 1);
 #pragma unroll
-                for (int s = 0; s < 8; ++s)
-                    fa[u][i][s] = *reinterpret_cast<const long*>(T + row * 256 + 32 * s + 8 * g);
+                for (int j = 0; j < 4; ++j) {
+                    const uint4 v =
+                        *reinterpret_cast<const uint4*>(T + row * 256 + 64 * j + 16 * g);
+                    fa[u][i][2 * j] = (long)(((uint64_t)v.y << 32) | v.x);
+                    fa[u][i][2 * j + 1] = (long)(((uint64_t)v.w << 32) | v.z);
+                }
             }
+        int cur_sl = -1;   // MAXQ: the output row of the run being folded
+        _Float16 cur = 0;  // and its running maximum (table row threadIdx.x)
         for (int q0 = 0; q0 < G; q0 += 16) {
             f32x4 acc[PROP_SUB][2];
 #pragma unroll
@@ -609,7 +649,8 @@ __global__ __launch_bounds__(256) void k_propose(const uint8_t* __restrict__ T, 
                 for (int i = 0; i < 2; ++i) acc[u][i] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
             for (int s = 0; s < 8; ++s) {
-                const long fb = *reinterpret_cast<const long*>(&sQ[(q0 + r) * 256 + 32 * s + 8 * g]);
+                const long fb = *reinterpret_cast<const long*>(
+                    &sQ[(q0 + r) * 256 + 64 * (s >> 1) + 16 * g + 8 * (s & 1)]);
 #pragma unroll
                 for (int u = 0; u < PROP_SUB; ++u)
 #pragma unroll
@@ -631,12 +672,58 @@ __global__ __launch_bounds__(256) void k_propose(const uint8_t* __restrict__ T, 
                     *reinterpret_cast<uint2*>(
                         &sS[r * PROP_ROWS + 32 * PROP_SUB * w + 32 * u + 16 * i + 4 * g]) = v.u2;
                 }
-            __syncthreads();
+            // MAXQ: thread t folds table row t, which its own wave wrote (the
+            // wave's rows are 32 PROP_SUB w .. + 32 PROP_SUB), and a wave's LDS
+            // accesses complete in order: the waves need not wait for each other
+            if constexpr (MAXQ)
+                lds_wave_fence();
+            else
+                __syncthreads();
+            if constexpr (MAXQ) {
+                const int c = threadIdx.x;
+                _Float16 x[16];
 #pragma unroll
-            for (int part = 0; part < PROP_SUB; ++part) {
-                const int64_t row = b0 + 128 * part + 8 * oc;
-                _Float16* dst = S + (int64_t)(q0 + oq) * ldS + row;
-                const _Float16* src = &sS[oq * PROP_ROWS + 128 * part + 8 * oc];
+                for (int q = 0; q < 16; ++q) x[q] = sS[q * PROP_ROWS + c];
+#pragma unroll
+                for (int q = 0; q < 16; ++q) {
+                    const int row = q0 + q;  // the run structure is uniform
+                    if (row < slots.lo || row >= slots.hi) continue;
+                    if ((slots.starts >> row) & 1) {  // a run starts: the last one is done
+                        if (cur_sl >= 0 && cur_sl < PROP_MAX_SLOTS) sM[cur_sl * PROP_ROWS + c] = cur;
+                        ++cur_sl;
+                        cur = x[q];
+                    } else {
+                        cur = x[q] > cur ? x[q] : cur;
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int part = 0; part < PROP_SUB; ++part) {
+                    const int64_t row = b0 + 128 * part + 8 * oc;
+                    _Float16* dst = S + (int64_t)(q0 + oq) * ldS + row;
+                    const _Float16* src = &sS[oq * PROP_ROWS + 128 * part + 8 * oc];
+                    if (vec && row + 8 <= n) {
+                        *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(src);
+                    } else {
+                        for (int x = 0; x < 8; ++x)
+                            if (row + x < n) dst[x] = src[x];
+                    }
+                }
+            }
+            if constexpr (MAXQ)
+                lds_wave_fence();  // the fold's reads before the next tile's writes
+            else
+                __syncthreads();
+        }
+        if constexpr (MAXQ) {
+            if (cur_sl >= 0 && cur_sl < PROP_MAX_SLOTS) sM[cur_sl * PROP_ROWS + threadIdx.x] = cur;
+            __syncthreads();
+            // every output row's PROP_ROWS scores, 16 bytes per thread and pass
+            for (int e = threadIdx.x; e < nslot * (PROP_ROWS / 8); e += 256) {
+                const int sl = e / (PROP_ROWS / 8), c8 = e % (PROP_ROWS / 8);
+                const int64_t row = b0 + 8 * c8;
+                _Float16* dst = S + (int64_t)sl * ldS + row;
+                const _Float16* src = &sM[sl * PROP_ROWS + 8 * c8];
                 if (vec && row + 8 <= n) {
                     *reinterpret_cast<uint4*>(dst) = *reinterpret_cast<const uint4*>(src);
                 } else {
@@ -852,7 +939,30 @@ extern "C" int sseqs_propose(const uint8_t* table, int64_t n, const uint8_t* q8,
     if (ld < n) return fail(-1, "propose: ld smaller than n");
     const unsigned grid =
         (unsigned)(ceil_div(n, PROP_ROWS) < 8192 ? ceil_div(n, PROP_ROWS) : 8192);
-    hipLaunchKernelGGL(k_propose, dim3(grid), dim3(256), 0, stream, table, n, q8, G,
-                       static_cast<_Float16*>(scores), ld);
+    hipLaunchKernelGGL(k_propose<false>, dim3(grid), dim3(256), 0, stream, table, n, q8, G,
+                       PropSlots{}, 0, static_cast<_Float16*>(scores), ld);
     return check_launch("propose");
+}
+
+extern "C" int sseqs_propose_max(const uint8_t* table, int64_t n, const uint8_t* q8, int G,
+                                 const int32_t* row_off, int nslot, void* scores, int64_t ld,
+                                 hipStream_t stream) {
+    if (n <= 0) return 0;
+    if (G <= 0 || G % 16 || G > 64) return fail(-1, "propose_max: G must be 16..64, a multiple of 16");
+    if (nslot < 1 || nslot > PROP_MAX_SLOTS) return fail(-1, "propose_max: 1..32 output rows");
+    if (!row_off) return fail(-1, "propose_max: no row offsets");
+    if (row_off[0] < 0 || row_off[nslot] > G)
+        return fail(-1, "propose_max: row offsets outside the G query rows");
+    PropSlots sl{0, row_off[0], row_off[nslot]};
+    for (int o = 0; o < nslot; ++o) {
+        if (row_off[o + 1] <= row_off[o])
+            return fail(-1, "propose_max: row offsets must increase (no empty output row)");
+        sl.starts |= 1ull << row_off[o];
+    }
+    if (ld < n) return fail(-1, "propose_max: ld smaller than n");
+    const unsigned grid =
+        (unsigned)(ceil_div(n, PROP_ROWS) < 8192 ? ceil_div(n, PROP_ROWS) : 8192);
+    hipLaunchKernelGGL(k_propose<true>, dim3(grid), dim3(256), 0, stream, table, n, q8, G, sl,
+                       nslot, static_cast<_Float16*>(scores), ld);
+    return check_launch("propose_max");
 }

@@ -616,6 +616,9 @@ class Proposer:
         self._scores = None  # [group, ld] scores of one launch's rows
         self._best = None    # [1, ld] a query's maximum over its rows
 
+    # queries of a launch group whose maxima the proposal kernel keeps itself
+    MAX_FUSED_QUERIES = 32
+
     def rows_of(self, queries: list[str]) -> tuple[list[str], np.ndarray]:
         """(embedding row texts, rows per query): each query whole, then,
         when longer than ``window``, its windows of ``window`` residues
@@ -634,7 +637,8 @@ class Proposer:
 
     def _groups(self, queries: list[str]) -> list[tuple]:
         """Launch groups: (query indices, q8 rows [Gp, 256], first row of
-        each query in the group, its row count).  Queries are packed first
+        each query in the group, its row count, and for a fused launch the
+        row offsets of its queries (int32, one more than queries) or None).  Queries are packed first
         fit by decreasing row count, so that few launches (each a read of the
         whole table) hold all rows; a query's rows stay in one group."""
         tab = self.table
@@ -667,7 +671,12 @@ class Proposer:
             q[: src.size] = e[torch.from_numpy(src).to(dev)]
             q8 = torch.empty((Gp, DIM), dtype=torch.uint8, device=dev)
             call("sseqs_enc_to_fp8", ptr(q), q.numel(), FP8_SCALE, ptr(q8), st)
-            groups.append((qidx, q8, loc, cnt))
+            # a group with a multi-row query that the fused launch holds
+            # writes one row per query (row j: the maximum over query j's rows)
+            off = None
+            if cnt.max() > 1 and qidx.size <= self.MAX_FUSED_QUERIES and Gp <= 64:
+                off = np.append(loc, cnt.sum()).astype(np.int32)
+            groups.append((qidx, q8, loc, cnt, off))
         return groups
 
     def _buffers(self, dev) -> int:
@@ -679,20 +688,33 @@ class Proposer:
             self._best = torch.empty((1, ld), dtype=torch.float16, device=dev)
         return ld
 
-    def _propose(self, q8: torch.Tensor, st) -> None:
+    def _propose(self, grp, st) -> None:
+        """Scores of one launch group into the score buffer: every row's, or
+        (fused groups) every query's maximum over its rows."""
         tab = self.table
-        call("sseqs_propose", ptr(tab.table), tab.n, ptr(q8), q8.shape[0], ptr(self._scores),
-             self._scores.shape[1], st)
-
-    def _query_row(self, first: int, count: int, out: int | None, st) -> int:
-        """Address of a query's score row: its only row, or the maximum over
-        its rows written to ``out`` (default: the scratch row)."""
+        qidx, q8, _, _, off = grp
         ld = self._scores.shape[1]
-        src = self._scores.data_ptr() + 2 * first * ld
-        if count == 1 and out is None:
+        if off is None:
+            call("sseqs_propose", ptr(tab.table), tab.n, ptr(q8), q8.shape[0],
+                 ptr(self._scores), ld, st)
+        else:
+            call("sseqs_propose_max", ptr(tab.table), tab.n, ptr(q8), q8.shape[0],
+                 off.ctypes.data, int(qidx.size), ptr(self._scores), ld, st)
+
+    def _query_row(self, grp, j: int, out: int | None, st) -> int:
+        """Address of query j's score row of the group after ``_propose``:
+        its row in the score buffer, or the maximum over its rows written to
+        ``out`` (default: the scratch row)."""
+        _, _, first, count, off = grp
+        ld = self._scores.shape[1]
+        if off is not None:
+            src, nrows = self._scores.data_ptr() + 2 * j * ld, 1
+        else:
+            src, nrows = self._scores.data_ptr() + 2 * int(first[j]) * ld, int(count[j])
+        if nrows == 1 and out is None:
             return src
         dst = out if out is not None else self._best.data_ptr()
-        call("sseqs_rows_max", src, ld, int(count), self.table.n, dst, st)
+        call("sseqs_rows_max", src, ld, nrows, self.table.n, dst, st)
         return dst
 
     def scores_for(self, queries: list[str]) -> torch.Tensor:
@@ -703,10 +725,10 @@ class Proposer:
         st = stream_of(dev)
         ld = self._buffers(dev)
         out = torch.empty((len(queries), ld), dtype=torch.float16, device=dev)
-        for qidx, q8, first, count in self._groups(queries):
-            self._propose(q8, st)
-            for j, i in enumerate(qidx):
-                self._query_row(int(first[j]), int(count[j]), out[int(i)].data_ptr(), st)
+        for grp in self._groups(queries):
+            self._propose(grp, st)
+            for j, i in enumerate(grp[0]):
+                self._query_row(grp, j, out[int(i)].data_ptr(), st)
         return out[:, : tab.n]
 
     # query-length classes of the calibrated operating point
@@ -800,14 +822,15 @@ class Proposer:
         st = stream_of(ms.dev)
         if getattr(ms, "learned_groups", None) is None:
             self.prepare(ms)
-        for qidx, q8, first, count in ms.learned_groups:
+        for grp in ms.learned_groups:
+            qidx = grp[0]
             if not ok[qidx].any():
                 continue
-            self._propose(q8, st)
+            self._propose(grp, st)
             for j, i in enumerate(qidx):
                 i = int(i)
                 if ok[i]:
-                    src = self._query_row(int(first[j]), int(count[j]), None, st)
+                    src = self._query_row(grp, j, None, st)
                     call("sseqs_topk", src, _lib.F16, tab.n, int(ms.kc_q[i]), 0,
                          ms.cand.data_ptr() + 8 * int(ms.c_off_h[i]), ptr(ms.sel_ws),
                          ms.sel_ws.numel(), st)

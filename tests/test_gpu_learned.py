@@ -215,8 +215,12 @@ def test_table_and_proposal(orc, enc, dev):
     assert list(count) == [1, 1, 3] and rows[3] == queries[2][:384] and rows[4] == queries[2][-384:]
     ms = MultiSearch(db, queries, top_prefilter=600, top_hits=50, proposer=prop16)
     ms.run()
-    sc_rows = prop16._scores[:5, : db.n].float().cpu()
     qe = enc.embed_strings(rows)
+    q8r = torch.zeros((16, 256), dtype=torch.uint8, device=dev)
+    call("sseqs_enc_to_fp8", ptr(qe), qe.numel(), FP8_SCALE, ptr(q8r), stream_of(dev))
+    raw_rows = torch.empty((16, db.n), dtype=torch.float16, device=dev)
+    call("sseqs_propose", ptr(tab.table), db.n, ptr(q8r), 16, ptr(raw_rows), db.n, stream_of(dev))
+    sc_rows = raw_rows[:5].float().cpu()
     qq = (FP8_SCALE * qe).cpu().to(torch.float8_e4m3fn).float()
     tt = tab.table.cpu().view(torch.float8_e4m3fn).float()
     ref = qq @ tt.T
@@ -300,3 +304,38 @@ def test_table_and_proposal(orc, enc, dev):
         assert torch.equal(mixed.cand[c], ref.cand[c]), i
         assert torch.equal(mixed.cs[c], ref.cs[c]), i
     assert cal1["k_by_len"] == cal["k_by_len"]
+
+
+def test_fused_proposal_max_equals_rows_max(dev):
+    """sseqs_propose_max against sseqs_propose + sseqs_rows_max, bit for bit:
+    runs of 1 to 7 query rows with padding rows before and after them, on a
+    table of 2.2 M rows (more than one table step per block) and on one of
+    1003 rows (ragged tail, unpadded stride with scalar stores)."""
+    from sseqs_amd._lib import call, ptr, stream_of
+    announce("fused proposal maximum vs proposal + row maximum")
+    st = stream_of(dev)
+    g = torch.Generator(device="cpu").manual_seed(41)
+    for n, G, runs in ((2_200_003, 64, [1, 6, 1, 1, 5, 2, 7, 3, 1, 1, 4, 6, 1, 2, 3, 2, 1, 1, 1,
+                                         1, 3, 5]),
+                       (1003, 48, [1] * 20 + [2] * 6 + [3, 4, 1, 1, 1, 1])):
+        table = (torch.randn((n, 256), generator=g) * 0.08).to(torch.float8_e4m3fn)
+        table = table.view(torch.uint8).to(dev)
+        q = (torch.randn((G, 256), generator=g) * 0.08).to(torch.float8_e4m3fn)
+        q8 = q.view(torch.uint8).to(dev)
+        off = np.concatenate([[0], np.cumsum(runs)]).astype(np.int32) + 1  # row 0 is padding
+        assert off[-1] < G and len(runs) <= 32
+        ld = n if n < 10_000 else (n + 7) // 8 * 8
+        raw = torch.empty((G, ld), dtype=torch.float16, device=dev)
+        call("sseqs_propose", ptr(table), n, ptr(q8), G, ptr(raw), ld, st)
+        fused = torch.full((len(runs) + 1, ld), 7.0, dtype=torch.float16, device=dev)
+        call("sseqs_propose_max", ptr(table), n, ptr(q8), G, off.ctypes.data, len(runs),
+             ptr(fused), ld, st)
+        ld8 = (n + 7) // 8 * 8
+        for o in range(len(runs)):
+            src = raw[off[o]: off[o + 1]]
+            src = src.contiguous() if ld == ld8 else torch.nn.functional.pad(src, (0, ld8 - ld))
+            want = torch.empty(ld8, dtype=torch.float16, device=dev)
+            call("sseqs_rows_max", ptr(src), ld8, int(runs[o]), n, ptr(want), st)
+            assert torch.equal(fused[o, :n].view(torch.int16), want[:n].view(torch.int16)), (n, o)
+        assert torch.all(fused[len(runs)] == 7.0)  # rows past the output rows untouched
+        del table, raw, fused

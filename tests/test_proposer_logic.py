@@ -54,3 +54,18 @@ def test_calibrated_counts_and_gate():
     assert list(p.meets_gate(qs, [100_000, 1, 1, 100_000, 1])) == [True, False, False, True, False]
     p.enforce_gate = False
     assert p.meets_gate(qs, [1] * 5).all()
+
+
+def test_fused_proposal_arguments_checked():
+    """sseqs_propose_max rejects its bad arguments on the host, before any
+    launch: query rows not a multiple of 16 or above 64, more than 32 output
+    rows, offsets that do not increase or leave the G rows."""
+    import ctypes as C
+
+    from sseqs_amd._lib import SseqsError, call
+    for G, off in ((80, [0, 1]), (24, [0, 1]), (64, list(range(34))), (64, [0, 3, 3]),
+                   (64, [0, 5, 2]), (32, [0, 40]), (32, [-1, 4])):
+        arr = np.array(off, np.int32)
+        with pytest.raises(SseqsError, match="propose_max: "):
+            call("sseqs_propose_max", C.c_void_p(16), 16, C.c_void_p(16), G,
+                 arr.ctypes.data, len(off) - 1, C.c_void_p(16), 16, None)
