@@ -44,6 +44,7 @@ TUNING_PATH = Path(__file__).resolve().parent / "tuning.json"
 
 # stripe-width classes that share SW queue parameters
 SW_STREAMS = 2  # streams the SW queues of different stripe widths alternate over
+BT_STREAMS = 2  # streams the queries' backtracks alternate over (a workspace each)
 SW_CLASSES = ((1, 8), (9, 16), (17, 32))
 DEFAULT_SW = {"item": 1024, "blocks_per_cu": 2}
 N_CU = 256
@@ -167,7 +168,10 @@ class MultiSearch:
         self.backtrack = bool(backtrack)
         if self.backtrack:
             lq_max = max(len(t) for t in self.texts)
-            self.bt_ws = BacktrackWorkspace(self.kh, lq_max, int(hit_width), dev)
+            nbt = BT_STREAMS if self.nq > 1 else 1
+            self.bt_wss = [BacktrackWorkspace(self.kh, lq_max, int(hit_width), dev)
+                           for _ in range(nbt)]
+            self.bt_ws = self.bt_wss[0]
             A = self.bt_ws.A
             self.out_q = torch.zeros((self.nq, self.kh, A), dtype=torch.uint8, device=dev)
             self.out_t = torch.zeros((self.nq, self.kh, A), dtype=torch.uint8, device=dev)
@@ -181,6 +185,10 @@ class MultiSearch:
         self._sw_streams = [torch.cuda.Stream(device=dev) for _ in range(SW_STREAMS)]
         self._sw_fork = torch.cuda.Event()
         self._sw_join = [torch.cuda.Event() for _ in self._sw_streams]
+        # the backtracks run after SW has joined: they reuse the SW streams
+        self._bt_streams = [self._sw_streams[i % SW_STREAMS] for i in range(BT_STREAMS)]
+        self._bt_fork = torch.cuda.Event()
+        self._bt_join = [torch.cuda.Event() for _ in self._bt_streams]
         self.graph: torch.cuda.CUDAGraph | None = None
 
     def set_sw_tuning(self, tuning: dict) -> None:
@@ -264,15 +272,32 @@ class MultiSearch:
         torch.index_select(self.cs, 0, self.hit_pos, out=self.hit_sc)
 
     def _backtrack(self) -> None:
+        """Gather + backtrack of every query's hits.  With several queries
+        the queries alternate over BT_STREAMS streams forked from and joined
+        back to the current stream, each with its own workspace: one
+        query's launch holds four waves per SIMD, and its longest hits are
+        its tail, so two queries in flight fill the GPU better."""
         if not self.backtrack:
             return
         from .search import backtrack_hits
+        main = torch.cuda.current_stream(self.dev)
+        side = self._bt_streams[: len(self.bt_wss)] if len(self.bt_wss) > 1 else []
+        if side:
+            self._bt_fork.record(main)
+            for s_ in side:
+                s_.wait_event(self._bt_fork)
         for i, q in enumerate(self.qs):
-            al = backtrack_hits(self.db, q, self.hit_ids[i * self.kh:(i + 1) * self.kh],
-                                self.bt_ws, self.go, self.ge)
-            self.out_q[i, :, : al.width].copy_(al.out_q)
-            self.out_t[i, :, : al.width].copy_(al.out_t)
-            self.info[i].copy_(al.info)
+            k = i % len(side) if side else 0
+            with torch.cuda.stream(side[k] if side else main):
+                al = backtrack_hits(self.db, q, self.hit_ids[i * self.kh:(i + 1) * self.kh],
+                                    self.bt_wss[k], self.go, self.ge)
+                self.out_q[i, :, : al.width].copy_(al.out_q)
+                self.out_t[i, :, : al.width].copy_(al.out_t)
+                self.info[i].copy_(al.info)
+        if side:
+            for s_, e in zip(side, self._bt_join):
+                e.record(s_)
+                main.wait_event(e)
 
     def run(self) -> None:
         """One sweep, launched eagerly on the current stream."""

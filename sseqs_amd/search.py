@@ -142,6 +142,7 @@ class BacktrackWorkspace:
         self.out_q = torch.zeros((n_max, self.A), dtype=torch.uint8, device=dev)
         self.out_t = torch.zeros((n_max, self.A), dtype=torch.uint8, device=dev)
         self.info = torch.zeros((n_max, 6), dtype=torch.int32, device=dev)
+        self.cell_off = torch.zeros(n_max, dtype=torch.int64, device=dev)
         total = lq_max * n_max * width
         nbytes = _lib.load().sseqs_backtrack_ws_bytes(lq_max, total, n_max)
         self.ws = _lib.workspace(nbytes, dev)
@@ -166,7 +167,7 @@ def backtrack_hits(db: SeqDB, q: PreparedQuery, ids: torch.Tensor,
     ids = ids.to(torch.int64).contiguous()
     call("sseqs_db_gather", ptr(db.packed), ptr(db.seq_off), ptr(db.seq_len), ptr(ids), n, W,
          ptr(ws.rows), ptr(ws.lens), st)
-    cell_off = ws.t_off[:n] * Lq
+    cell_off = torch.mul(ws.t_off[:n], Lq, out=ws.cell_off[:n])  # no allocation
     total = Lq * n * W
     call("sseqs_backtrack", ptr(q.qbytes), Lq, ptr(ws.rows), ptr(ws.t_off),
          ptr(ws.lens), ptr(cell_off), total, n, int(gap_open), int(gap_extend), ws.A,

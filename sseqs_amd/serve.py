@@ -39,7 +39,7 @@ import torch
 
 from . import _lib
 from .db import SeqDB
-from .multi import MultiSearch
+from .multi import BT_STREAMS, MultiSearch
 
 
 @dataclass
@@ -71,12 +71,13 @@ def query_bytes(db: SeqDB, q_len: int, top_prefilter: int, top_hits: int,
 
 def sweep_bytes(db: SeqDB, lq_max: int, top_hits: int, hit_width: int = 500) -> int:
     """Device bytes a sweep holds besides its queries: one prefilter score
-    vector, the select workspace and the backtrack workspace."""
+    vector, the select workspace and the backtrack workspaces (one per
+    backtrack stream)."""
     lib = _lib.load()
     kh = top_hits
-    return (db.n * 4 + int(lib.sseqs_topk_ws_bytes(db.n)) +
-            int(lib.sseqs_backtrack_ws_bytes(lq_max, lq_max * kh * hit_width, kh)) +
-            kh * (hit_width + lq_max + 1) * 2)
+    bt = (int(lib.sseqs_backtrack_ws_bytes(lq_max, lq_max * kh * hit_width, kh)) +
+          kh * (hit_width + lq_max + 1) * 2 + kh * (hit_width + 8))
+    return db.n * 4 + int(lib.sseqs_topk_ws_bytes(db.n)) + BT_STREAMS * bt
 
 
 class SearchEngine:
