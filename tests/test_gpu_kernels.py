@@ -442,6 +442,30 @@ def test_topk(orc, dev, dtype, n, k):
     assert np.array_equal(got_u, np.sort(want))
 
 
+@pytest.mark.parametrize("offset", [0, 3])
+def test_topk_f16_ragged_and_unaligned(orc, dev, offset):
+    """fp16 scores take eight per thread from 16-byte loads when the
+    address is 16-byte aligned, one per thread otherwise: both equal the
+    oracle, for a length that ends mid-vector and mid-tile, with heavy ties
+    at the k-th score."""
+    from sseqs_amd import ops
+    announce(f"topk fp16 offset={offset}")
+    rng = np.random.default_rng(99 + offset)
+    n = 1_000_003
+    x = np.round(rng.standard_normal(n + offset), 1).astype(np.float16)
+    x[x == 0] = 0.0  # no -0.0: the keys order it below +0.0, the oracle ties them
+    x[rng.integers(0, n, n // 50)] = -np.inf
+    base = torch.from_numpy(x).to(dev)
+    t = base[offset:]
+    xs = x[offset:].astype(np.float64)
+    for k in (1, 4096, 100_000, n):
+        want = orc.topk_desc(xs, k)
+        got = ops.topk_indices(t, k, sorted=False).cpu().numpy()
+        assert np.array_equal(got, np.sort(want)), k
+    got = ops.topk_indices(t, 5000, sorted=True).cpu().numpy()
+    assert np.array_equal(got, orc.topk_desc(xs, 5000))
+
+
 def test_topk_errors(dev):
     from sseqs_amd import ops
     t = torch.zeros(10, device=dev)
