@@ -18,8 +18,9 @@ the encoder itself.
 Every database sequence is embedded once into an HBM-resident table of
 256 FP8 (e4m3) values per sequence (51.2 GB for 200 M sequences).  A batch
 of queries is embedded the same way and the proposal GEMM
-(``sseqs_propose``, MFMA fp8) scores it against the whole table; each
-query's top ``top_prefilter`` go to exact affine SW, which stays the judge.
+(``sseqs_propose`` / ``sseqs_propose_max``, MFMA fp8) scores it against the
+whole table; each query's top ``top_prefilter`` go to exact affine SW, which
+alone decides the hits.
 ``Proposer`` plugs this into ``multi.MultiSearch`` (and so into the serving
 engine); its ``calibrate`` measures Recall@4096 against exhaustive SW on
 sample queries and sets the candidate count per query length that reaches
