@@ -12,6 +12,14 @@ The metric's query length is 200 (``value``); the same cascade with the
 SW (-4 per gap position) in place of the affine stage under
 ``q500_linear4``.
 
+The same line carries ``batched``: 256 queries of U[50, 1000] residues in one
+sweep (HIP-graph replay, per-stage eager times, the dominant prefilter
+launch's roofline), and ``learned``: the FP8 embedding table of the whole
+database, the same sweep with the learned proposal (the proposal launch's
+HBM roofline), Recall@4096 of both prefilters against exhaustive SW, the
+calibrated 0.99-recall operating point and the continuous-batching engine
+under Poisson arrivals (p50/p99 latency, achieved requests/s).
+
 Prints one JSON line.  ``python bench.py --gpus N`` with N > 1 exits: the
 build targets one GPU.
 """
