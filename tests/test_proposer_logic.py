@@ -69,3 +69,66 @@ def test_fused_proposal_arguments_checked():
         with pytest.raises(SseqsError, match="propose_max: "):
             call("sseqs_propose_max", C.c_void_p(16), 16, C.c_void_p(16), G,
                  arr.ctypes.data, len(off) - 1, C.c_void_p(16), 16, None)
+
+
+def test_launch_groups(monkeypatch):
+    """Launch groups on the host (device calls stubbed): every query in one
+    group with its rows contiguous; at most `group` rows per group, first
+    fit by decreasing row count; a group with a windowed query and at most 32
+    queries (and at most 64 rows) gets the fused launch's row offsets, which
+    are its queries' row runs, strictly increasing; other groups get none."""
+    import torch
+
+    from sseqs_amd import learned
+
+    staged = []  # the fp32 query rows handed to the fp8 conversion, per group
+
+    def fake_call(name, *args):
+        if name == "sseqs_enc_to_fp8":
+            staged.append(args[0].clone())
+
+    monkeypatch.setattr(learned, "call", fake_call)
+    monkeypatch.setattr(learned, "ptr", lambda t: t)
+    monkeypatch.setattr(learned, "stream_of", lambda dev: 0)
+    enc = SimpleNamespace(embed_strings=lambda rows: torch.arange(
+        len(rows), dtype=torch.float32)[:, None].repeat(1, learned.DIM))
+    tab = SimpleNamespace(n=1_000_000, table=torch.zeros(1, dtype=torch.uint8), enc=enc)
+    rng = np.random.default_rng(3)
+    lens = list(rng.integers(30, 1000, 150)) + [50] * 60
+    queries = ["".join("ACDEFGHIKLMNPQRSTVWY"[(i + j) % 20] for j in range(L))
+               for i, L in enumerate(lens)]
+    for group in (64, 128):
+        p = Proposer(tab, group=group)
+        rows, count = p.rows_of(queries)
+        first = np.concatenate([[0], np.cumsum(count)[:-1]])
+        staged.clear()
+        groups = p._groups(queries)
+        assert len(staged) == len(groups)
+        seen = np.concatenate([g[0] for g in groups])
+        assert np.array_equal(np.sort(seen), np.arange(len(queries)))
+        fused = 0
+        for (qidx, q8, loc, cnt, off), q in zip(groups, staged):
+            assert np.array_equal(cnt, count[qidx]) and int(cnt.sum()) <= group
+            # the group's rows: each query's rows in order (the stub embeds
+            # row r as the value r), then zero padding to a multiple of 16
+            want = np.concatenate([first[i] + np.arange(count[i]) for i in qidx])
+            assert np.array_equal(q[: want.size, 0].numpy(), want.astype(np.float32))
+            assert not q[want.size:].any()
+            assert q8.shape == ((int(cnt.sum()) + 15) // 16 * 16, learned.DIM)
+            assert np.array_equal(loc, np.concatenate([[0], np.cumsum(cnt)[:-1]]))
+            want_fused = cnt.max() > 1 and qidx.size <= 32 and q8.shape[0] <= 64
+            assert (off is not None) == want_fused
+            if off is not None:
+                fused += 1
+                assert off.dtype == np.int32 and off.size == qidx.size + 1
+                assert np.array_equal(off[:-1], loc) and off[-1] == cnt.sum()
+                assert np.all(np.diff(off) > 0)
+        assert fused > 0 or group == 128
+        assert sum(int(g[3].sum()) for g in groups) == int(count.sum()) == len(rows)
+        # first fit by decreasing row count: a query did not fit any earlier
+        # group when it was placed, so a later group's first query (its
+        # largest) would overflow every earlier group's final fill
+        fill = [int(g[3].sum()) for g in groups]
+        for j in range(1, len(groups)):
+            head = int(groups[j][3].max())
+            assert all(fill[i] + head > group for i in range(j)), (group, j)
