@@ -322,7 +322,7 @@ int sseqs_enc_db_rows(const uint8_t* packed, const int64_t* seq_off, const int32
 int sseqs_enc_to_fp8(const float* e, int64_t n, float scale, uint8_t* out, hipStream_t stream);
 /* scores[q * ld + s] (fp16) = sum_c table[s][c] * q8[q][c] for the n rows of
  * the fp8 table (256 bytes each) and G query rows (G a multiple of 16, at
- * most 128), on MFMA fp8. */
+ * most 128), on MFMA fp8.  table and q8 are 16-byte aligned. */
 int sseqs_propose(const uint8_t* table, int64_t n, const uint8_t* q8, int G, void* scores,
                   int64_t ld, hipStream_t stream);
 /* out[j] = max over r < nrows of S[r * ld + j] for j < n (fp16; ld >= n and a

@@ -937,6 +937,8 @@ extern "C" int sseqs_propose(const uint8_t* table, int64_t n, const uint8_t* q8,
     if (n <= 0) return 0;
     if (G <= 0 || G % 16 || G > 128) return fail(-1, "propose: G must be 16..128, a multiple of 16");
     if (ld < n) return fail(-1, "propose: ld smaller than n");
+    if ((reinterpret_cast<uintptr_t>(table) | reinterpret_cast<uintptr_t>(q8)) & 15)
+        return fail(-1, "propose: table and query rows must be 16-byte aligned");
     const unsigned grid =
         (unsigned)(ceil_div(n, PROP_ROWS) < 8192 ? ceil_div(n, PROP_ROWS) : 8192);
     hipLaunchKernelGGL(k_propose<false>, dim3(grid), dim3(256), 0, stream, table, n, q8, G,
@@ -960,6 +962,8 @@ extern "C" int sseqs_propose_max(const uint8_t* table, int64_t n, const uint8_t*
         sl.starts |= 1ull << row_off[o];
     }
     if (ld < n) return fail(-1, "propose_max: ld smaller than n");
+    if ((reinterpret_cast<uintptr_t>(table) | reinterpret_cast<uintptr_t>(q8)) & 15)
+        return fail(-1, "propose_max: table and query rows must be 16-byte aligned");
     const unsigned grid =
         (unsigned)(ceil_div(n, PROP_ROWS) < 8192 ? ceil_div(n, PROP_ROWS) : 8192);
     hipLaunchKernelGGL(k_propose<true>, dim3(grid), dim3(256), 0, stream, table, n, q8, G, sl,

@@ -59,7 +59,8 @@ def test_calibrated_counts_and_gate():
 def test_fused_proposal_arguments_checked():
     """sseqs_propose_max rejects its bad arguments on the host, before any
     launch: query rows not a multiple of 16 or above 64, more than 32 output
-    rows, offsets that do not increase or leave the G rows."""
+    rows, offsets that do not increase or leave the G rows; both proposal
+    entry points reject a table or query rows not 16-byte aligned."""
     import ctypes as C
 
     from sseqs_amd._lib import SseqsError, call
@@ -69,6 +70,15 @@ def test_fused_proposal_arguments_checked():
         with pytest.raises(SseqsError, match="propose_max: "):
             call("sseqs_propose_max", C.c_void_p(16), 16, C.c_void_p(16), G,
                  arr.ctypes.data, len(off) - 1, C.c_void_p(16), 16, None)
+    # operands the 16-byte loads cannot take
+    ok = np.array([0, 3], np.int32)
+    for table, q8 in ((17, 16), (16, 24)):
+        with pytest.raises(SseqsError, match="16-byte aligned"):
+            call("sseqs_propose_max", C.c_void_p(table), 16, C.c_void_p(q8), 16, ok.ctypes.data, 1,
+                 C.c_void_p(16), 16, None)
+        with pytest.raises(SseqsError, match="16-byte aligned"):
+            call("sseqs_propose", C.c_void_p(table), 16, C.c_void_p(q8), 16, C.c_void_p(16), 16,
+                 None)
 
 
 def test_launch_groups(monkeypatch):
